@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""Benchmark: D3Q27 MLUPS and HBM-roofline fraction on one MI355X, FP16-shifted vs FP32 storage,
+plus the adjoint/forward time ratio.  Prints ONE JSON line.
+
+Workload (the largest single-GPU configuration): D3Q27 BGK, 512x256x256 nodes (33.5 M),
+z-spanning cylinder D=40 at (128,128), Re=300, u=0.05, half-way bounce-back on the body and the
+four side walls, Zou-He inlet, zero-gradient outlet, drag/lift every 10 steps.  Synthetic
+initial state (seeded perturbation); the populations are resident in HBM before timing starts.
+
+A "step" is one fused lattice update of the whole domain.  MLUPS = nodes * steps / seconds over
+exactly --steps steps bracketed by stream synchronisation.  The roofline uses the algorithmic
+2*Q*sizeof(storage) bytes per node update and the fused kernel's average launch time measured
+with HIP events on the solver's own stream (a second context with forces off, so only that
+kernel runs in the timed region).
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "cylinder-flow_amd")
+HBM_PEAK_GBS = 8000.0
+METRIC = "D3Q27 MLUPS + % HBM roofline (FP32 vs FP16-store); adjoint/forward time"
+
+
+def load_package():
+    if "cylinder_flow_amd" in sys.modules:
+        return sys.modules["cylinder_flow_amd"]
+    spec = importlib.util.spec_from_file_location("cylinder_flow_amd", os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["cylinder_flow_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_leg(pkg, precision, steps, warmup, force_interval):
+    W = pkg.configs.cfg2_zcyl3d(precision=precision, force_interval=force_interval)
+    s = pkg.Solver(W.cfg).initialize()
+    s.step(warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    info = s.info()
+    return s, W, wall, info
+
+
+def kernel_roofline(pkg, precision, steps, bytes_per_update, nodes):
+    """Average duration of the fused kernel alone (forces off), HIP events on its stream."""
+    W = pkg.configs.cfg2_zcyl3d(precision=precision, force_interval=0)
+    s = pkg.Solver(W.cfg).initialize()
+    s.step(5)
+    ms = s.step_timed(steps)
+    s.close()
+    per_launch = ms / steps / 1e3
+    achieved = bytes_per_update * nodes / per_launch / 1e9
+    return per_launch, achieved
+
+
+def traffic_from_profiles(name):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh).get(name)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(seconds=15.0):
+    """The FP64 CPU oracle (push-form port of the plain-mode step, OpenMP) on a slab of the same
+    workload: 512x256x32 nodes with the same cylinder, walls, inlet and outlet."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    nx, ny, nz = 512, 256, 32
+    u, D, Re = 0.05, 40.0, 300.0
+    cfg = O.make_cfg(27, nx, ny, nz, tau0=3 * u * D / Re, u0=u, body=O.BODY_ZCYL, center=(128.0, 128.0, 0.0),
+                     r2=(D / 2) ** 2, perturb=1e-3, force_interval=10)
+    st = O.PlainState(cfg)
+    st.step(1)
+    t0 = time.perf_counter()
+    st.step(1)
+    one = time.perf_counter() - t0
+    n = max(2, int(seconds / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    st.step(n)
+    dt = time.perf_counter() - t0
+    mlups = nx * ny * nz * n / dt / 1e6
+    return {"value": round(mlups, 2), "unit": "MLUPS", "cores": O.lib().orc_num_threads(), "kind": "port",
+            "sample": f"FP64 oracle, {n} steps of a 512x256x32 slab of the workload ({dt:.1f} s)"}
+
+
+def adjoint_ratio(pkg, steps=500):
+    """Adjoint (recompute + reverse sweep) over forward time on the D3Q27 adjoint workload's
+    192x96x96 geometry, FP32, window [steps/2, steps], checkpoint every 250 steps."""
+    W = pkg.configs.cfg4_adjoint3d()
+    s = pkg.Solver(W.cfg)
+    s.set_solid_fraction(W.extra["sfrac"])
+    s.initialize()
+    J, ds, du, tf, ta = s.adjoint(steps // 2, steps, W.cd_scale(), 250)
+    s.close()
+    return {"workload": "cfg4_d3q27_adjoint_192x96x96 (fp32), %d steps, checkpoint 250" % steps,
+            "forward_ms": round(tf, 3), "adjoint_ms": round(ta, 3), "ratio": round(ta / tf, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the whole domain runs on a single MI355X; "
+              "multi-GPU runs are not part of this build (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    pkg = load_package()
+    L = pkg.lbm
+    K, Wm = a.steps, a.warmup
+
+    log(f"fp16s leg: {K} steps after {Wm} warm-up")
+    s16, W16, wall16, info16 = run_leg(pkg, L.FP16S, K, Wm, 10)
+    nodes = info16.nodes
+    forces = s16.forces()
+    s16.close()
+    mlups16 = nodes * K / wall16 / 1e6
+    log(f"fp16s {mlups16:.0f} MLUPS, {wall16 / K * 1e3:.4f} ms/step; kernel roofline run")
+    t16, ach16 = kernel_roofline(pkg, L.FP16S, K, info16.bytes_per_update, nodes)
+    out = {
+        "metric": METRIC, "value": round(mlups16, 1), "unit": "MLUPS", "n_gpus": 1, "steps": K, "warmup": Wm,
+        "ms_per_step": round(wall16 / K * 1e3, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": W16.name, "lattice": "D3Q27", "nx": 512, "ny": 256, "nz": 256,
+                   "storage": "fp16 shifted (f_i - w_i rho0)/w_i", "collision": "fp32",
+                   "force_interval": 10, "body": "z-cylinder D=40", "Re": 300},
+        "roofline": {"bound": "hbm", "achieved": round(ach16, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach16 / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles("fp16s"),
+                     "kernel": "k_fused<27,SF16,false,false,false>", "kernel_ms": round(t16 * 1e3, 5),
+                     "bytes_per_update": info16.bytes_per_update},
+        "forces_finite": bool(forces.size and (abs(forces[:, 1:]) < 1e30).all()),
+    }
+    if not a.no_fp32:
+        log("fp32 leg")
+        s32, W32, wall32, info32 = run_leg(pkg, L.FP32, K, Wm, 10)
+        s32.close()
+        t32, ach32 = kernel_roofline(pkg, L.FP32, K, info32.bytes_per_update, nodes)
+        out["fp32"] = {"value": round(nodes * K / wall32 / 1e6, 1), "unit": "MLUPS",
+                       "ms_per_step": round(wall32 / K * 1e3, 5),
+                       "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS,
+                                    "unit": "GB/s", "frac": round(ach32 / HBM_PEAK_GBS, 4),
+                                    "traffic": traffic_from_profiles("fp32"),
+                                    "kernel": "k_fused<27,SF32,false,false,false>",
+                                    "kernel_ms": round(t32 * 1e3, 5), "bytes_per_update": info32.bytes_per_update}}
+    if not a.no_adjoint:
+        log("adjoint leg")
+        out["adjoint"] = adjoint_ratio(pkg)
+    if not a.no_cpu:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

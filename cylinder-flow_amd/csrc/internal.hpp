@@ -1,0 +1,131 @@
+// internal.hpp -- context and launcher interfaces shared by the kernel files and the
+// C ABI (lbm_capi.cpp).  Not installed; include/lbm_capi.h is the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/lbm_capi.h"
+#include "store.hpp"
+
+namespace lbm {
+
+// Device layout of the plain path: SoA, one plane per direction, x fastest, rows padded
+// to `nxp` (a multiple of 64 elements) so every wavefront row segment starts aligned.
+//   element (i, x, y, z) = i*plane + x + nxp*(y + ny*z),   plane = nxp*ny*nz
+struct Geo {
+    int nx, ny, nz, nxp;
+    size_t plane;
+    int xper, yper, zper;
+};
+
+struct FusedArgs {
+    Geo g;
+    double tau;      // tau0 + 0.5 (FP64 path: f - (f - feq)/tau, the reference's form)
+    float omega;     // 1/(tau0 + 0.5) (FP32 path)
+    StoreParams sp;
+};
+
+// node flags
+enum : uint8_t { FL_FLUID = 0, FL_SOLID = 1, FL_NEAR = 2 /* fluid with a solid neighbour */ };
+
+// launch shape of every per-node kernel: blocks of `bx` threads along x, `gx` per row
+struct Shape {
+    int bx, gx;
+};
+Shape node_shape(const Geo &g);
+
+hipError_t plain_init(int Q, int prec, void *A, void *B, double *macros, const uint8_t *flags, const double *uin,
+                      const double *xi, double perturb, uint64_t seed, const FusedArgs &a, hipStream_t s);
+hipError_t plain_step(int Q, int prec, const void *A, void *B, const uint8_t *flags, const void *sfrac,
+                      const double *uin, const int *block_slot, double *partials, double *macros,
+                      bool force, bool macros_on, const FusedArgs &a, hipStream_t s);
+hipError_t force_reduce(const double *partials, int nslots, double *hist, int rec, long step, hipStream_t s);
+// FP64 <-> storage conversion of one direction plane (physical nodes, x fastest)
+hipError_t plain_decode(int Q, int prec, const void *A, double *out, const uint8_t *flags, int dir,
+                        const FusedArgs &a, hipStream_t s);
+hipError_t plain_encode(int Q, int prec, void *A, const double *in, const uint8_t *flags, int dir,
+                        const FusedArgs &a, hipStream_t s);
+hipError_t macros_extract(const double *macros, double *out, int k, const Geo &g, hipStream_t s);
+
+// ---- reference mode (FP64, ghost-padded SoA) ----
+struct RefGeo {
+    int Q, D, g;                 // ghost width 3 (3D) / 2 (2D)
+    int glx, gly, glz;           // interior extents
+    int lx, ly, lz;              // padded extents
+    size_t plane;                // lx*ly*lz
+    double tau0, u0;
+};
+hipError_t ref_init(const RefGeo &r, double *f, double *ft, double *mac, uint8_t *usr, const int32_t *ran,
+                    int body, double cx, double cy, double cz, double r2, uint64_t seed, hipStream_t s);
+hipError_t ref_fills(const RefGeo &r, double *f, hipStream_t s);
+hipError_t ref_usrbc(const RefGeo &r, double *f, const uint8_t *usr, hipStream_t s);
+hipError_t ref_mem(const RefGeo &r, const double *f, const uint8_t *usr, double *partials, int nblk,
+                   double *hist, int rec, long step, hipStream_t s);
+hipError_t ref_fused(const RefGeo &r, const double *f, double *ft, double *mac, const uint8_t *usr,
+                     const double *prof, hipStream_t s);
+int ref_mem_blocks(const RefGeo &r);
+
+// ---- adjoint (plain mode) ----
+struct AdjArgs {
+    FusedArgs f;
+    double alpha;        // window weight of this step (dJ/dFx)
+    double alpha_next;   // window weight of the next step
+};
+hipError_t adj_step(int Q, int prec, const void *H, const void *mu_next, void *mu, const uint8_t *flags,
+                    const void *sfrac, const double *uin, void *dJds, double *dJdu_rows, const AdjArgs &a,
+                    hipStream_t s);
+hipError_t adj_init(int Q, int prec, const void *mu1, const uint8_t *flags, const double *uin,
+                    const double *xi, double perturb, uint64_t seed, double alpha1, double *dJdu_rows,
+                    const FusedArgs &a, hipStream_t s);
+hipError_t adj_reduce_ds(int prec, const void *dJds, double *out, const Geo &g, hipStream_t s);
+
+}  // namespace lbm
+
+struct SnapshotJob;
+
+struct lbm_ctx {
+    lbm_config cfg{};
+    int Q = 0, D = 0, prec = 0;
+    lbm::Geo geo{};
+    size_t N = 0;           // physical nodes
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // host-side set-up inputs
+    std::vector<uint8_t> solid;     // N
+    std::vector<double> sfrac;      // N or empty
+    std::vector<double> uin;        // ny*nz
+    std::vector<double> xi;         // N or empty (injected perturbation)
+    std::vector<int32_t> ran;       // padded box or empty (reference mode)
+    bool uin_set = false;
+    int solid_nodes = 0, near_nodes = 0;
+
+    // plain-mode device state
+    void *A = nullptr, *B = nullptr;
+    uint8_t *flags = nullptr;
+    void *dsfrac = nullptr;
+    double *duin = nullptr;
+    double *macros = nullptr;       // 4 planes of FP64
+    int *block_slot = nullptr;
+    double *partials = nullptr;
+    int nslots = 0;
+
+    // force history (device + host copy)
+    double *hist = nullptr;
+    int hist_cap = 0, nrec = 0, nrec_checked = 0;
+
+    // reference-mode device state
+    lbm::RefGeo rg{};
+    double *rf = nullptr, *rft = nullptr, *rmac = nullptr, *rprof = nullptr;
+    uint8_t *rusr = nullptr;
+
+    bool initialized = false;
+    bool macros_valid = false;
+    long step = 0;
+
+    SnapshotJob *snap = nullptr;
+};

@@ -1,0 +1,665 @@
+// lbm_capi.cpp -- implementation of include/lbm_capi.h (host side; kernels live in *.hip).
+//
+// The context owns every device allocation and one non-blocking HIP stream (default
+// priority).  The reference's set-up (System::System + LBM::LBM + LBM::Initialize,
+// 3D MPI+KOKKOS+MPIIO/System.cpp:5-23, lbm.hpp:146-219, lbm.cpp:36-256) becomes
+// lbm_create + lbm_initialize; its time loop body becomes lbm_step.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "internal.hpp"
+#include "lattice.hpp"
+
+using namespace lbm;
+
+static thread_local std::string t_create_err;
+
+static int set_err(lbm_ctx *c, int code, const std::string &m) {
+    if (c) c->err = m;
+    else t_create_err = m;
+    return code;
+}
+
+#define HIP_TRY(c, expr)                                                                               \
+    do {                                                                                               \
+        hipError_t e__ = (expr);                                                                       \
+        if (e__ != hipSuccess)                                                                         \
+            return set_err((c), LBM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e__));     \
+    } while (0)
+
+template <int Q>
+static void table(int *e, double *w, int *opp) {
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = I;
+        if (e) {
+            e[3 * i] = Lat<Q>::ex(i);
+            e[3 * i + 1] = Lat<Q>::ey(i);
+            e[3 * i + 2] = Lat<Q>::ez(i);
+        }
+        if (w) w[i] = Lat<Q>::w(i);
+        if (opp) opp[i] = Lat<Q>::opp(i);
+    });
+}
+
+extern "C" int lbm_lattice_table(int lattice, int *e, double *w, int *opp) {
+    if (lattice == 9) table<9>(e, w, opp);
+    else if (lattice == 27) table<27>(e, w, opp);
+    else return LBM_ERR_INVALID;
+    return LBM_OK;
+}
+
+extern "C" int lbm_config_default(lbm_config *c, int lattice) {
+    if (!c || (lattice != 9 && lattice != 27)) return LBM_ERR_INVALID;
+    std::memset(c, 0, sizeof(*c));
+    const double cs2 = 1.0 / 3.0;
+    c->lattice = lattice;
+    c->rho0 = 1.0;
+    c->precision = LBM_FP64;
+    c->bc_mode = LBM_BC_REFERENCE;
+    c->body = LBM_BODY_SPHERE;
+    c->inlet = LBM_INLET_UNIFORM;
+    c->seed = 5374857ull;   // the reference's Random_XorShift64_Pool seed
+    if (lattice == 27) {
+        // input.in: rho R Re / u0 Time inter / nx ny nz = 1.0 8.0 100 / 0.1 .. / 512 128 128
+        const double R = 8.0, Re = 100.0;
+        c->nx = 512; c->ny = 128; c->nz = 128;
+        c->u0 = 0.1;
+        c->tau0 = c->u0 * R / Re / cs2;                       // System.cpp:19
+        c->cx = c->cy = c->cz = (double)((c->nx - 1) / 8);    // lbm.cpp:238 (integer division)
+        const double r = (c->nx - 1) / 64.0;
+        c->r2 = r * r;
+        c->force_interval = 0;                                // the 3D reference logs no force
+    } else {
+        // MPI Kokkos/System.cpp:9-17 and main.cpp:8-9: nx 512, ny 128, R = int(0.1*ny)
+        c->nx = 512; c->ny = 128; c->nz = 1;
+        c->u0 = 0.1;
+        const int R = (int)(0.1 * c->ny);
+        c->tau0 = c->u0 * R / 100.0 / cs2;
+        c->cx = 0.5 * (c->nx - 1);                            // lbm.cpp:108
+        c->cy = 0.5 * (c->ny - 1);
+        const double r = 0.03125 * (c->nx - 1) / 2.0;
+        c->r2 = r * r;
+        c->force_interval = 1;                                // test.dat every step, lbm.cpp:398-407
+    }
+    return LBM_OK;
+}
+
+extern "C" const char *lbm_last_error(const lbm_ctx *ctx) { return ctx ? ctx->err.c_str() : t_create_err.c_str(); }
+
+static void free_all(lbm_ctx *c) {
+    void *ptrs[] = {c->A, c->B, c->flags, c->dsfrac, c->duin, c->macros, c->block_slot, c->partials, c->hist,
+                    c->rf, c->rft, c->rmac, c->rprof, c->rusr};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    c->A = c->B = nullptr;
+    c->flags = nullptr;
+    c->dsfrac = nullptr;
+    c->duin = c->macros = c->partials = c->hist = nullptr;
+    c->block_slot = nullptr;
+    c->rf = c->rft = c->rmac = c->rprof = nullptr;
+    c->rusr = nullptr;
+}
+
+extern "C" int lbm_create(const lbm_config *cfg, lbm_ctx **out) {
+    if (!cfg || !out) return set_err(nullptr, LBM_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const int Q = cfg->lattice;
+    if (Q != 9 && Q != 27) return set_err(nullptr, LBM_ERR_INVALID, "lattice must be 9 (D2Q9) or 27 (D3Q27)");
+    const int nz = Q == 9 ? 1 : cfg->nz;
+    if (cfg->nx < 3 || cfg->ny < 3 || (Q == 27 && nz < 3) || (Q == 9 && cfg->nz > 1))
+        return set_err(nullptr, LBM_ERR_INVALID, "extents: need nx, ny (and nz for D3Q27) >= 3, nz = 1 for D2Q9");
+    if (!(cfg->tau0 > 0.0) || !std::isfinite(cfg->tau0))
+        return set_err(nullptr, LBM_ERR_INVALID, "tau0 must be > 0 (relaxation time tau0 + 0.5 > 0.5)");
+    if (cfg->precision < LBM_FP64 || cfg->precision > LBM_BF16S)
+        return set_err(nullptr, LBM_ERR_INVALID, "unknown precision");
+    if (cfg->bc_mode != LBM_BC_REFERENCE && cfg->bc_mode != LBM_BC_PLAIN)
+        return set_err(nullptr, LBM_ERR_INVALID, "unknown bc_mode");
+    if (cfg->bc_mode == LBM_BC_REFERENCE && cfg->precision != LBM_FP64)
+        return set_err(nullptr, LBM_ERR_INVALID, "reference boundary mode runs in FP64 only");
+    if (cfg->body < LBM_BODY_NONE || cfg->body > LBM_BODY_MASK)
+        return set_err(nullptr, LBM_ERR_INVALID, "unknown body");
+    if (cfg->bc_mode == LBM_BC_REFERENCE && cfg->body == LBM_BODY_MASK)
+        return set_err(nullptr, LBM_ERR_INVALID, "reference mode takes its body from the sphere/cylinder formula");
+    if (cfg->inlet < LBM_INLET_UNIFORM || cfg->inlet > LBM_INLET_PROFILE)
+        return set_err(nullptr, LBM_ERR_INVALID, "unknown inlet");
+    if (cfg->ncoef < 0 || cfg->ncoef > 8) return set_err(nullptr, LBM_ERR_INVALID, "ncoef must be in [0, 8]");
+    if (cfg->force_interval < 0) return set_err(nullptr, LBM_ERR_INVALID, "force_interval must be >= 0");
+    const int nxp = (cfg->nx + 63) / 64 * 64;
+    const size_t plane = (size_t)nxp * cfg->ny * nz;
+    if (plane >= (size_t)1 << 32) return set_err(nullptr, LBM_ERR_INVALID, "domain too large: a direction plane must stay below 2^32 nodes");
+
+    lbm_ctx *c = new lbm_ctx();
+    c->cfg = *cfg;
+    c->cfg.nz = nz;
+    c->Q = Q;
+    c->D = Q == 9 ? 2 : 3;
+    c->prec = cfg->precision;
+    c->geo.nx = cfg->nx;
+    c->geo.ny = cfg->ny;
+    c->geo.nz = nz;
+    c->geo.nxp = nxp;
+    c->geo.plane = plane;
+    const bool plain = cfg->bc_mode == LBM_BC_PLAIN;
+    c->geo.xper = plain && cfg->periodic[0];
+    c->geo.yper = plain && cfg->periodic[1];
+    c->geo.zper = plain && Q == 27 && cfg->periodic[2];
+    c->N = (size_t)cfg->nx * cfg->ny * nz;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        set_err(nullptr, LBM_ERR_HIP, std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
+        delete c;
+        return LBM_ERR_HIP;
+    }
+    *out = c;
+    return LBM_OK;
+}
+
+int snapshot_wait(lbm_ctx *c);
+
+extern "C" void lbm_destroy(lbm_ctx *c) {
+    if (!c) return;
+    snapshot_wait(c);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int lbm_set_mask(lbm_ctx *c, const uint8_t *solid) {
+    if (!c || !solid) return set_err(c, LBM_ERR_INVALID, "null argument");
+    if (c->initialized) return set_err(c, LBM_ERR_STATE, "lbm_set_mask after lbm_initialize");
+    if (c->cfg.body != LBM_BODY_MASK) return set_err(c, LBM_ERR_INVALID, "lbm_set_mask requires body = LBM_BODY_MASK");
+    c->solid.assign(solid, solid + c->N);
+    return LBM_OK;
+}
+
+extern "C" int lbm_set_solid_fraction(lbm_ctx *c, const double *s) {
+    if (!c || !s) return set_err(c, LBM_ERR_INVALID, "null argument");
+    if (c->initialized) return set_err(c, LBM_ERR_STATE, "lbm_set_solid_fraction after lbm_initialize");
+    if (c->cfg.bc_mode != LBM_BC_PLAIN) return set_err(c, LBM_ERR_INVALID, "partial bounce-back is a plain-mode feature");
+    for (size_t n = 0; n < c->N; n++)
+        if (!(s[n] >= 0.0 && s[n] <= 1.0)) return set_err(c, LBM_ERR_INVALID, "solid fraction outside [0, 1]");
+    c->sfrac.assign(s, s + c->N);
+    return LBM_OK;
+}
+
+extern "C" int lbm_set_inlet_profile(lbm_ctx *c, const double *uin) {
+    if (!c || !uin) return set_err(c, LBM_ERR_INVALID, "null argument");
+    if (c->initialized) return set_err(c, LBM_ERR_STATE, "lbm_set_inlet_profile after lbm_initialize");
+    c->uin.assign(uin, uin + (size_t)c->geo.ny * c->geo.nz);
+    c->uin_set = true;
+    return LBM_OK;
+}
+
+extern "C" int lbm_set_perturbation(lbm_ctx *c, const double *xi) {
+    if (!c || !xi) return set_err(c, LBM_ERR_INVALID, "null argument");
+    if (c->initialized) return set_err(c, LBM_ERR_STATE, "lbm_set_perturbation after lbm_initialize");
+    c->xi.assign(xi, xi + c->N);
+    return LBM_OK;
+}
+
+static void ref_extents(const lbm_ctx *c, RefGeo &r) {
+    r.Q = c->Q;
+    r.D = c->D;
+    r.g = c->D == 3 ? 3 : 2;   // ghost width: 3D MPI+KOKKOS+MPIIO/lbm.hpp:22, MPI Kokkos/lbm.hpp:79-80
+    r.glx = c->geo.nx;
+    r.gly = c->geo.ny;
+    r.glz = c->geo.nz;
+    r.lx = r.glx + 2 * r.g;
+    r.ly = r.gly + 2 * r.g;
+    r.lz = c->D == 3 ? r.glz + 2 * r.g : 1;
+    r.plane = (size_t)r.lx * r.ly * r.lz;
+    r.tau0 = c->cfg.tau0;
+    r.u0 = c->cfg.u0;
+}
+
+extern "C" int lbm_set_ran(lbm_ctx *c, const int32_t *ran, size_t count) {
+    if (!c || !ran) return set_err(c, LBM_ERR_INVALID, "null argument");
+    if (c->initialized) return set_err(c, LBM_ERR_STATE, "lbm_set_ran after lbm_initialize");
+    if (c->cfg.bc_mode != LBM_BC_REFERENCE) return set_err(c, LBM_ERR_INVALID, "ran is the reference-mode perturbation");
+    RefGeo r;
+    ref_extents(c, r);
+    if (count != r.plane) return set_err(c, LBM_ERR_INVALID, "ran must cover the ghost-padded box lx*ly*lz");
+    for (size_t n = 0; n < count; n++)
+        if (ran[n] < 0 || ran[n] > 99) return set_err(c, LBM_ERR_INVALID, "ran values must lie in [0, 100)");
+    c->ran.assign(ran, ran + count);
+    return LBM_OK;
+}
+
+// ---- host-side geometry ----
+static double legendre(int k, double x) {
+    double p0 = 1.0, p1 = x;
+    if (k == 0) return p0;
+    for (int n = 1; n < k; n++) {
+        const double p2 = ((2.0 * n + 1.0) * x * p1 - (double)n * p0) / (double)(n + 1);
+        p0 = p1;
+        p1 = p2;
+    }
+    return p1;
+}
+
+// inlet velocity u_x(y, z); walls sit half a spacing outside the first/last node
+static void inlet_profile(const lbm_ctx *c, std::vector<double> &uin) {
+    static const int LY[8] = {0, 1, 0, 2, 1, 0, 3, 0}, LZ[8] = {0, 0, 1, 0, 1, 2, 0, 3};
+    const int ny = c->geo.ny, nz = c->geo.nz, D = c->D;
+    uin.assign((size_t)ny * nz, 0.0);
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++) {
+            const double ey = ((double)y + 0.5) / (double)ny, ez = ((double)z + 0.5) / (double)nz;
+            const double by = 4.0 * ey * (1.0 - ey), bz = D == 3 ? 4.0 * ez * (1.0 - ez) : 1.0;
+            double u;
+            if (c->cfg.inlet == LBM_INLET_PARABOLIC) {
+                u = c->cfg.u0 * by * bz;
+            } else if (c->cfg.inlet == LBM_INLET_LEGENDRE) {
+                double sum = 0.0;
+                for (int k = 0; k < c->cfg.ncoef; k++) {
+                    const double phi = D == 3 ? legendre(LY[k], 2.0 * ey - 1.0) * legendre(LZ[k], 2.0 * ez - 1.0)
+                                              : legendre(k, 2.0 * ey - 1.0);
+                    sum += c->cfg.coef[k] * phi;
+                }
+                u = by * bz * sum;
+            } else {
+                u = c->cfg.u0;
+            }
+            uin[y + (size_t)ny * z] = u;
+        }
+}
+
+static bool body_inside(const lbm_config &cfg, int D, int x, int y, int z) {
+    if (cfg.body != LBM_BODY_SPHERE && cfg.body != LBM_BODY_ZCYL) return false;
+    const double dx = (double)x - cfg.cx, dy = (double)y - cfg.cy;
+    double r = dx * dx + dy * dy;
+    if (D == 3 && cfg.body == LBM_BODY_SPHERE) {
+        const double dz = (double)z - cfg.cz;
+        r = r + dz * dz;
+    }
+    return r <= cfg.r2;
+}
+
+static int grow_hist(lbm_ctx *c, int need) {
+    if (need <= c->hist_cap) return LBM_OK;
+    int cap = std::max(need, std::max(1024, 2 * c->hist_cap));
+    double *h = nullptr;
+    HIP_TRY(c, hipMalloc(&h, (size_t)cap * 4 * sizeof(double)));
+    if (c->hist) {
+        HIP_TRY(c, hipMemcpyAsync(h, c->hist, (size_t)c->nrec * 4 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        (void)hipFree(c->hist);
+    }
+    c->hist = h;
+    c->hist_cap = cap;
+    return LBM_OK;
+}
+
+FusedArgs ctx_fused_args(const lbm_ctx *c) {
+    FusedArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.g = c->geo;
+    a.tau = c->cfg.tau0 + 0.5;
+    a.omega = (float)(1.0 / (c->cfg.tau0 + 0.5));
+    // 16-bit formats: offset o_i = feq_i(0, (u_ref, 0, 0)) and o_i / w_i
+    std::vector<int> e(3 * c->Q), opp(c->Q);
+    std::vector<double> w(c->Q);
+    lbm_lattice_table(c->Q, e.data(), w.data(), opp.data());
+    const double U = c->cfg.store_uref;
+    for (int i = 0; i < c->Q; i++) {
+        const double eu = e[3 * i] * U;
+        const double o = w[i] * (3.0 * eu + 4.5 * eu * eu - 1.5 * U * U);
+        a.sp.off[i] = (float)o;
+        a.sp.offw[i] = (float)(o / w[i]);
+    }
+    return a;
+}
+
+static int init_plain(lbm_ctx *c) {
+    const Geo &g = c->geo;
+    const int Q = c->Q, D = c->D;
+    // solid mask: the body formula (the reference's "usr", 3D MPI+KOKKOS+MPIIO/lbm.cpp:238) or a user mask
+    if (c->cfg.body == LBM_BODY_MASK) {
+        if (c->solid.size() != c->N) return set_err(c, LBM_ERR_STATE, "body = LBM_BODY_MASK needs lbm_set_mask first");
+    } else {
+        c->solid.assign(c->N, 0);
+        for (int z = 0; z < g.nz; z++)
+            for (int y = 0; y < g.ny; y++)
+                for (int x = 0; x < g.nx; x++)
+                    c->solid[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] = body_inside(c->cfg, D, x, y, z);
+    }
+    if (!c->uin_set) inlet_profile(c, c->uin);
+    // node flags (padded layout); padding columns are marked solid
+    std::vector<int> e(3 * Q);
+    lbm_lattice_table(Q, e.data(), nullptr, nullptr);
+    std::vector<uint8_t> flags(g.plane, FL_SOLID);
+    auto solid_at = [&](int x, int y, int z) -> bool {
+        if (g.xper) x = (x + g.nx) % g.nx;
+        if (g.yper) y = (y + g.ny) % g.ny;
+        if (g.zper) z = (z + g.nz) % g.nz;
+        if (x < 0 || x >= g.nx || y < 0 || y >= g.ny || z < 0 || z >= g.nz) return false;
+        return c->solid[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] != 0;
+    };
+    c->solid_nodes = c->near_nodes = 0;
+    for (int z = 0; z < g.nz; z++)
+        for (int y = 0; y < g.ny; y++)
+            for (int x = 0; x < g.nx; x++) {
+                const size_t node = (size_t)x + (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
+                if (solid_at(x, y, z)) {
+                    flags[node] = FL_SOLID;
+                    c->solid_nodes++;
+                    continue;
+                }
+                bool near = false;
+                for (int i = 1; i < Q && !near; i++) near = solid_at(x - e[3 * i], y - e[3 * i + 1], z - e[3 * i + 2]);
+                flags[node] = near ? FL_NEAR : FL_FLUID;
+                c->near_nodes += near;
+            }
+    // force partial slots: blocks holding a node next to the mask or with a solid fraction
+    const Shape sh = node_shape(g);
+    const size_t nblocks = (size_t)sh.gx * g.ny * g.nz;
+    std::vector<int> slot(nblocks, -1);
+    c->nslots = 0;
+    const bool sf = !c->sfrac.empty();
+    for (int z = 0; z < g.nz; z++)
+        for (int y = 0; y < g.ny; y++)
+            for (int bxi = 0; bxi < sh.gx; bxi++) {
+                bool any = false;
+                for (int x = bxi * sh.bx; x < std::min(g.nx, (bxi + 1) * sh.bx) && !any; x++) {
+                    const size_t node = (size_t)x + (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
+                    any = flags[node] == FL_NEAR ||
+                          (sf && flags[node] != FL_SOLID && c->sfrac[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] > 0.0);
+                }
+                if (any) slot[(size_t)bxi + (size_t)sh.gx * ((size_t)y + (size_t)g.ny * z)] = c->nslots++;
+            }
+    const size_t esz = c->prec == LBM_FP64 ? 8 : c->prec == LBM_FP32 ? 4 : 2;
+    HIP_TRY(c, hipMalloc(&c->A, esz * Q * g.plane));
+    HIP_TRY(c, hipMalloc(&c->B, esz * Q * g.plane));
+    HIP_TRY(c, hipMalloc(&c->flags, g.plane));
+    HIP_TRY(c, hipMalloc(&c->duin, sizeof(double) * c->uin.size()));
+    HIP_TRY(c, hipMalloc(&c->macros, sizeof(double) * 4 * g.plane));
+    HIP_TRY(c, hipMalloc(&c->block_slot, sizeof(int) * nblocks));
+    HIP_TRY(c, hipMalloc(&c->partials, sizeof(double) * 3 * std::max(c->nslots, 1)));
+    HIP_TRY(c, hipMemcpyAsync(c->flags, flags.data(), g.plane, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->duin, c->uin.data(), sizeof(double) * c->uin.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->block_slot, slot.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, c->stream));
+    if (sf) {
+        const size_t ssz = c->prec == LBM_FP64 ? 8 : 4;
+        std::vector<char> buf(ssz * g.plane, 0);
+        for (int z = 0; z < g.nz; z++)
+            for (int y = 0; y < g.ny; y++)
+                for (int x = 0; x < g.nx; x++) {
+                    const size_t node = (size_t)x + (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
+                    const double v = c->sfrac[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)];
+                    if (ssz == 8) ((double *)buf.data())[node] = v;
+                    else ((float *)buf.data())[node] = (float)v;
+                }
+        HIP_TRY(c, hipMalloc(&c->dsfrac, buf.size()));
+        HIP_TRY(c, hipMemcpyAsync(c->dsfrac, buf.data(), buf.size(), hipMemcpyHostToDevice, c->stream));
+    }
+    double *dxi = nullptr;
+    if (!c->xi.empty()) {
+        HIP_TRY(c, hipMalloc(&dxi, sizeof(double) * c->N));
+        HIP_TRY(c, hipMemcpyAsync(dxi, c->xi.data(), sizeof(double) * c->N, hipMemcpyHostToDevice, c->stream));
+    }
+    const FusedArgs a = ctx_fused_args(c);
+    HIP_TRY(c, plain_init(Q, c->prec, c->A, c->B, c->macros, c->flags, c->duin, dxi, c->cfg.perturb, c->cfg.seed, a, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (dxi) (void)hipFree(dxi);
+    return LBM_OK;
+}
+
+static int init_ref(lbm_ctx *c) {
+    RefGeo &r = c->rg;
+    ref_extents(c, r);
+    const int Q = c->Q;
+    HIP_TRY(c, hipMalloc(&c->rf, sizeof(double) * Q * r.plane));
+    HIP_TRY(c, hipMalloc(&c->rft, sizeof(double) * Q * r.plane));
+    HIP_TRY(c, hipMalloc(&c->rmac, sizeof(double) * 4 * r.plane));
+    HIP_TRY(c, hipMalloc(&c->rusr, r.plane));
+    const int nprof = c->D == 3 ? r.glz : r.gly;
+    std::vector<double> prof(nprof);
+    for (int k = 0; k < nprof; k++)   // u0*4*c*(n-1-c)/(n-1)^2, lbm.cpp:405 / MPI Kokkos/lbm.cpp:446
+        prof[k] = c->cfg.u0 * 4.0 * (double)k * (double)(nprof - 1 - k) / ((double)(nprof - 1) * (double)(nprof - 1));
+    HIP_TRY(c, hipMalloc(&c->rprof, sizeof(double) * nprof));
+    HIP_TRY(c, hipMemcpyAsync(c->rprof, prof.data(), sizeof(double) * nprof, hipMemcpyHostToDevice, c->stream));
+    int32_t *dran = nullptr;
+    if (!c->ran.empty()) {
+        HIP_TRY(c, hipMalloc(&dran, sizeof(int32_t) * r.plane));
+        HIP_TRY(c, hipMemcpyAsync(dran, c->ran.data(), sizeof(int32_t) * r.plane, hipMemcpyHostToDevice, c->stream));
+    }
+    HIP_TRY(c, ref_init(r, c->rf, c->rft, c->rmac, c->rusr, dran, c->cfg.body, c->cfg.cx, c->cfg.cy, c->cfg.cz,
+                        c->cfg.r2, c->cfg.seed, c->stream));
+    std::vector<uint8_t> usr(r.plane);
+    HIP_TRY(c, hipMemcpyAsync(usr.data(), c->rusr, r.plane, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (dran) (void)hipFree(dran);
+    // usrbc works in place; reject masks for which its result would depend on thread order
+    // (a solid node, a fluid gap of one node and another solid node along a link)
+    std::vector<int> e(3 * Q);
+    lbm_lattice_table(Q, e.data(), nullptr, nullptr);
+    const int g = r.g, klo = c->D == 3 ? g : 0, khi = c->D == 3 ? r.lz - g : 1;
+    c->solid_nodes = 0;
+    for (int k = klo; k < khi; k++)
+        for (int j = g; j < r.ly - g; j++)
+            for (int i = g; i < r.lx - g; i++) {
+                const long n = i + (long)r.lx * (j + (long)r.ly * k);
+                if (usr[n] != 0) continue;
+                c->solid_nodes++;
+                for (int q = 1; q < Q; q++) {
+                    const long d = e[3 * q] + (long)r.lx * (e[3 * q + 1] + (long)r.ly * e[3 * q + 2]);
+                    if (usr[n + d] == 1 && usr[n + 2 * d] == 0)
+                        return set_err(c, LBM_ERR_INVALID, "reference mode: body has a one-node fluid gap along a lattice link (usrbc would race)");
+                }
+            }
+    const int nblk = ref_mem_blocks(r);
+    HIP_TRY(c, hipMalloc(&c->partials, sizeof(double) * 3 * nblk));
+    c->nslots = nblk;
+    return LBM_OK;
+}
+
+extern "C" int lbm_initialize(lbm_ctx *c) {
+    if (!c) return LBM_ERR_INVALID;
+    if (c->initialized) return set_err(c, LBM_ERR_STATE, "already initialised");
+    int rc = c->cfg.bc_mode == LBM_BC_PLAIN ? init_plain(c) : init_ref(c);
+    if (rc != LBM_OK) {
+        free_all(c);
+        return rc;
+    }
+    c->initialized = true;
+    c->macros_valid = true;
+    c->step = 0;
+    c->nrec = c->nrec_checked = 0;
+    return LBM_OK;
+}
+
+static int step_n(lbm_ctx *c, int n) {
+    const int fi = c->cfg.force_interval;
+    const int nforce = fi > 0 ? (int)((c->step + n) / fi - c->step / fi) : 0;
+    int rc = grow_hist(c, c->nrec + nforce + 1);
+    if (rc != LBM_OK) return rc;
+    if (c->cfg.bc_mode == LBM_BC_REFERENCE) {
+        for (int k = 0; k < n; k++) {
+            c->step++;
+            const bool force = fi > 0 && c->step % fi == 0;
+            HIP_TRY(c, ref_fills(c->rg, c->rf, c->stream));
+            HIP_TRY(c, ref_usrbc(c->rg, c->rf, c->rusr, c->stream));
+            if (force) HIP_TRY(c, ref_mem(c->rg, c->rf, c->rusr, c->partials, c->nslots, c->hist, c->nrec++, c->step, c->stream));
+            HIP_TRY(c, ref_fused(c->rg, c->rf, c->rft, c->rmac, c->rusr, c->rprof, c->stream));
+            std::swap(c->rf, c->rft);
+        }
+        return LBM_OK;
+    }
+    const FusedArgs a = ctx_fused_args(c);
+    for (int k = 0; k < n; k++) {
+        c->step++;
+        const bool force = fi > 0 && c->step % fi == 0;
+        HIP_TRY(c, plain_step(c->Q, c->prec, c->A, c->B, c->flags, c->dsfrac, c->duin, c->block_slot, c->partials,
+                              c->macros, force, k == n - 1, a, c->stream));
+        if (force) HIP_TRY(c, force_reduce(c->partials, c->nslots, c->hist, c->nrec++, c->step, c->stream));
+        std::swap(c->A, c->B);
+    }
+    return LBM_OK;
+}
+
+extern "C" int lbm_step(lbm_ctx *c, int n) {
+    if (!c) return LBM_ERR_INVALID;
+    if (!c->initialized) return set_err(c, LBM_ERR_STATE, "lbm_step before lbm_initialize");
+    if (n < 0) return set_err(c, LBM_ERR_INVALID, "negative step count");
+    if (n == 0) return LBM_OK;
+    return step_n(c, n);
+}
+
+extern "C" int lbm_step_timed(lbm_ctx *c, int n, double *ms) {
+    if (!c || !ms) return LBM_ERR_INVALID;
+    if (!c->initialized) return set_err(c, LBM_ERR_STATE, "lbm_step_timed before lbm_initialize");
+    hipEvent_t e0, e1;
+    HIP_TRY(c, hipEventCreate(&e0));
+    HIP_TRY(c, hipEventCreate(&e1));
+    HIP_TRY(c, hipEventRecord(e0, c->stream));
+    int rc = n > 0 ? step_n(c, n) : LBM_OK;
+    HIP_TRY(c, hipEventRecord(e1, c->stream));
+    HIP_TRY(c, hipEventSynchronize(e1));
+    float t = 0.f;
+    HIP_TRY(c, hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t;
+    return rc;
+}
+
+extern "C" int lbm_sync(lbm_ctx *c) {
+    if (!c) return LBM_ERR_INVALID;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->nrec > c->nrec_checked) {
+        const int k = c->nrec - c->nrec_checked;
+        std::vector<double> h(4 * (size_t)k);
+        HIP_TRY(c, hipMemcpy(h.data(), c->hist + 4 * (size_t)c->nrec_checked, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+        const int first = c->nrec_checked;
+        c->nrec_checked = c->nrec;
+        for (int r = 0; r < k; r++)
+            for (int d = 1; d < 4; d++) {
+                const double v = h[4 * (size_t)r + d];
+                if (!std::isfinite(v) || (c->cfg.force_limit > 0.0 && std::fabs(v) > c->cfg.force_limit))
+                    return set_err(c, LBM_ERR_UNSTABLE, "force record " + std::to_string(first + r) + " (step " +
+                                                          std::to_string((long)h[4 * (size_t)r]) + ") is not finite or exceeds force_limit");
+            }
+    }
+    return LBM_OK;
+}
+
+// strip the ghost layers of a reference-mode plane (x fastest, interior only)
+static void ref_strip(const lbm_ctx *c, const std::vector<double> &pad, double *out) {
+    const RefGeo &r = c->rg;
+    const int g = r.g, gz = c->D == 3 ? g : 0;
+    size_t o = 0;
+    for (int k = 0; k < r.glz; k++)
+        for (int j = 0; j < r.gly; j++)
+            for (int i = 0; i < r.glx; i++, o++)
+                out[o] = pad[(size_t)(i + g) + (size_t)r.lx * ((size_t)(j + g) + (size_t)r.ly * (k + gz))];
+}
+
+extern "C" int lbm_get_fields(lbm_ctx *c, double *p, double *u, double *v, double *w) {
+    if (!c) return LBM_ERR_INVALID;
+    if (!c->initialized) return set_err(c, LBM_ERR_STATE, "lbm_get_fields before lbm_initialize");
+    double *outs[4] = {p, u, v, w};
+    if (c->cfg.bc_mode == LBM_BC_REFERENCE) {
+        std::vector<double> pad(c->rg.plane);
+        for (int k = 0; k < 4; k++) {
+            if (!outs[k]) continue;
+            HIP_TRY(c, hipMemcpyAsync(pad.data(), c->rmac + (size_t)k * c->rg.plane, sizeof(double) * c->rg.plane,
+                                      hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            ref_strip(c, pad, outs[k]);
+        }
+        return LBM_OK;
+    }
+    double *tmp = nullptr;
+    HIP_TRY(c, hipMalloc(&tmp, sizeof(double) * c->N));
+    for (int k = 0; k < 4; k++) {
+        if (!outs[k]) continue;
+        hipError_t e = macros_extract(c->macros, tmp, k, c->geo, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(outs[k], tmp, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(tmp);
+            return set_err(c, LBM_ERR_HIP, std::string("lbm_get_fields: ") + hipGetErrorString(e));
+        }
+    }
+    (void)hipFree(tmp);
+    return LBM_OK;
+}
+
+extern "C" int lbm_get_populations(lbm_ctx *c, double *f) {
+    if (!c || !f) return set_err(c, LBM_ERR_INVALID, "null argument");
+    if (!c->initialized) return set_err(c, LBM_ERR_STATE, "lbm_get_populations before lbm_initialize");
+    if (c->cfg.bc_mode == LBM_BC_REFERENCE) {
+        std::vector<double> pad(c->rg.plane);
+        for (int i = 0; i < c->Q; i++) {
+            HIP_TRY(c, hipMemcpyAsync(pad.data(), c->rf + (size_t)i * c->rg.plane, sizeof(double) * c->rg.plane,
+                                      hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+            ref_strip(c, pad, f + (size_t)i * c->N);
+        }
+        return LBM_OK;
+    }
+    const FusedArgs a = ctx_fused_args(c);
+    double *tmp = nullptr;
+    HIP_TRY(c, hipMalloc(&tmp, sizeof(double) * c->N));
+    for (int i = 0; i < c->Q; i++) {
+        hipError_t e = plain_decode(c->Q, c->prec, c->A, tmp, c->flags, i, a, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(f + (size_t)i * c->N, tmp, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(tmp);
+            return set_err(c, LBM_ERR_HIP, std::string("lbm_get_populations: ") + hipGetErrorString(e));
+        }
+    }
+    (void)hipFree(tmp);
+    return LBM_OK;
+}
+
+extern "C" int lbm_set_populations(lbm_ctx *c, const double *f) {
+    if (!c || !f) return set_err(c, LBM_ERR_INVALID, "null argument");
+    if (!c->initialized) return set_err(c, LBM_ERR_STATE, "lbm_set_populations before lbm_initialize");
+    if (c->cfg.bc_mode == LBM_BC_REFERENCE)
+        return set_err(c, LBM_ERR_INVALID, "reference mode keeps ghost state; populations cannot be replaced");
+    const FusedArgs a = ctx_fused_args(c);
+    double *tmp = nullptr;
+    HIP_TRY(c, hipMalloc(&tmp, sizeof(double) * c->N));
+    for (int i = 0; i < c->Q; i++) {
+        hipError_t e = hipMemcpyAsync(tmp, f + (size_t)i * c->N, sizeof(double) * c->N, hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = plain_encode(c->Q, c->prec, c->A, tmp, c->flags, i, a, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(tmp);
+            return set_err(c, LBM_ERR_HIP, std::string("lbm_set_populations: ") + hipGetErrorString(e));
+        }
+    }
+    (void)hipFree(tmp);
+    return LBM_OK;
+}
+
+extern "C" int lbm_get_forces(lbm_ctx *c, double *out, int max, int *count) {
+    if (!c) return LBM_ERR_INVALID;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (count) *count = c->nrec;
+    const int n = std::min(max, c->nrec);
+    if (out && n > 0) HIP_TRY(c, hipMemcpy(out, c->hist, sizeof(double) * 4 * (size_t)n, hipMemcpyDeviceToHost));
+    return LBM_OK;
+}
+
+extern "C" int lbm_info_get(lbm_ctx *c, lbm_info *info) {
+    if (!c || !info) return LBM_ERR_INVALID;
+    std::memset(info, 0, sizeof(*info));
+    info->lattice = c->Q;
+    info->nx = c->geo.nx;
+    info->ny = c->geo.ny;
+    info->nz = c->geo.nz;
+    info->pitch = c->cfg.bc_mode == LBM_BC_PLAIN ? c->geo.nxp : c->rg.lx;
+    info->nodes = c->N;
+    const size_t esz = c->prec == LBM_FP64 ? 8 : c->prec == LBM_FP32 ? 4 : 2;
+    info->state_bytes = c->cfg.bc_mode == LBM_BC_PLAIN ? esz * c->Q * c->geo.plane : 8 * c->Q * c->rg.plane;
+    info->bytes_per_update = 2.0 * c->Q * (double)esz;
+    info->step = c->step;
+    info->solid_nodes = c->solid_nodes;
+    info->boundary_nodes = c->near_nodes;
+    return LBM_OK;
+}

@@ -1,0 +1,372 @@
+// plain_kernels.hip -- fused pull-scheme lattice update for gfx950 (plain boundary mode).
+//
+// One kernel per time step replaces the reference's Collision -> boundary fills -> usrbc
+// -> stream1 -> stream2 -> Update sequence (3D MPI+KOKKOS+MPIIO/lbm.cpp:257-417;
+// MPI Kokkos/lbm.cpp:127-139,313-457).  Each lane owns one node and
+//   1. pulls Q populations from buffer A (neighbour x - e_i, or a boundary rule),
+//   2. applies the Zou-He velocity inlet at x = 0,
+//   3. takes the moments p = sum g / 3, u = sum g e (the reference's Update),
+//   4. relaxes toward the shifted incompressible equilibrium (the reference's Collision,
+//      for the next step) and applies partial bounce-back where a solid fraction is set,
+//   5. writes Q populations to buffer B.
+// A and B swap every step, so the kernel never reads what it writes (race-free by
+// construction; the reference's in-place usrbc is not).  Global memory traffic per node is
+// Q reads + Q writes -- the 2*Q*sizeof(storage) bytes the roofline counts -- plus one byte of
+// node flags.
+//
+// Boundary rules of a pulled entry (node, i), first match wins:
+//   x == 0 and e_ix > 0 (inlet, x not periodic)   -> Zou-He unknown, computed in step 2
+//   source outside y/z (walls, not periodic)       -> half-way bounce-back g_i = h_opp(i)(node)
+//   source x == nx (outlet, x not periodic)        -> zero gradient, source clamped to nx-1
+//   source is solid (mask)                         -> half-way bounce-back, momentum exchange
+//   otherwise                                      -> g_i = h_i(node - e_i) (periodic wrap)
+#include "fused_common.hpp"
+
+namespace lbm {
+
+Shape node_shape(const Geo &g) {
+    Shape s;
+    s.bx = g.nxp <= 512 ? g.nxp : 256;
+    s.gx = (g.nxp + s.bx - 1) / s.bx;
+    return s;
+}
+
+template <int NW>
+__device__ __forceinline__ void block_sum3(double &a, double &b, double &c, double *red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+        c += __shfl_xor(c, off);
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[3 * wv] = a;
+        red[3 * wv + 1] = b;
+        red[3 * wv + 2] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        a = red[0]; b = red[1]; c = red[2];
+        for (int k = 1; k < nw; k++) {
+            a += red[3 * k];
+            b += red[3 * k + 1];
+            c += red[3 * k + 2];
+        }
+    }
+}
+
+template <int Q, class S>
+__global__ void __launch_bounds__(512) k_init(typename S::T *__restrict__ A, typename S::T *__restrict__ B,
+                                              double *__restrict__ macros, const uint8_t *__restrict__ flags, const double *__restrict__ uin,
+                                              const double *__restrict__ xi, double perturb, uint64_t seed,
+                                              const FusedArgs a) {
+    using L = Lat<Q>;
+    using T = typename S::T;
+    const Geo &g = a.g;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+    if (x >= g.nxp) return;
+    const size_t node = (size_t)x + (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
+    const bool live = x < g.nx && flags[node] != FL_SOLID;
+    double u = 0.0;
+    if (live) {
+        const size_t phys = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
+        const double r = xi ? xi[phys] : uniform_pm1(seed, phys);
+        u = uin[y + (size_t)g.ny * z] + perturb * r;
+    }
+    macros[node] = 0.0;
+    macros[g.plane + node] = u;
+    macros[2 * g.plane + node] = 0.0;
+    macros[3 * g.plane + node] = 0.0;
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = I;
+        const double f = live ? feq_ref<Q, i>(0.0, u, 0.0, 0.0) : 0.0;
+        const T h = S::encode(f, (float)(1.0 / L::w(i)), a.sp.offw[i]);
+        A[(size_t)i * g.plane + node] = h;
+        B[(size_t)i * g.plane + node] = h;
+    });
+}
+
+template <int Q, class S, bool FORCE, bool MACROS, bool SFRAC>
+__global__ void __launch_bounds__(512)
+    k_fused(const typename S::T *__restrict__ A, typename S::T *__restrict__ B, const uint8_t *__restrict__ flags,
+            const typename S::C *__restrict__ sfrac, const double *__restrict__ uin,
+            const int *__restrict__ block_slot, double *__restrict__ partials, double *__restrict__ macros,
+            const FusedArgs a) {
+    using L = Lat<Q>;
+    using T = typename S::T;
+    using C = typename S::C;
+    const Geo &g = a.g;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const bool active = x < g.nx;
+    // element offset inside one direction plane (a plane is < 2^32 elements)
+    const uint32_t node = (uint32_t)x + (uint32_t)g.nxp * ((uint32_t)y + (uint32_t)g.ny * (uint32_t)z);
+    const uint8_t fl = active ? flags[node] : (uint8_t)FL_SOLID;
+    const bool live = fl != FL_SOLID;
+    const bool near = fl == FL_NEAR;
+
+    C gv[Q];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+
+    // ---- 1. pull ----
+    pull<Q, S, FORCE>(A, flags, g, a.sp, x, y, z, node, live, near, gv, fx, fy, fz);
+
+    // ---- 2. Zou-He inlet ----
+    if (!g.xper && x == 0 && live) zou_he<Q, C>(gv, (C)uin[y + g.ny * z]);
+
+    // ---- 3./4. moments, collision, partial bounce-back, store ----
+    C sf = 0;
+    if constexpr (SFRAC) sf = live ? sfrac[node] : (C)0;
+    double mp, mu, mv, mw;
+    if constexpr (sizeof(C) == 8) {
+        // FP64: the reference's term order -- p += f/3.0 per direction (lbm.cpp:390-397)
+        double p = 0.0, u = 0.0, v = 0.0, w = 0.0;
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = I;
+            p = p + gv[i] / 3.0;
+            if constexpr (L::ex(i) > 0) u = u + gv[i];
+            else if constexpr (L::ex(i) < 0) u = u - gv[i];
+            if constexpr (L::ey(i) > 0) v = v + gv[i];
+            else if constexpr (L::ey(i) < 0) v = v - gv[i];
+            if constexpr (L::ez(i) > 0) w = w + gv[i];
+            else if constexpr (L::ez(i) < 0) w = w - gv[i];
+        });
+        if constexpr (FORCE && SFRAC) {
+            fx += 2.0 * sf * u;
+            fy += 2.0 * sf * v;
+            fz += 2.0 * sf * w;
+        }
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = I;
+            const double feq = feq_ref<Q, i>(p, u, v, w);
+            double c = gv[i] - (gv[i] - feq) / a.tau;
+            if constexpr (SFRAC) c = (1.0 - sf) * c + sf * gv[L::opp(i)];
+            if (live) S::template store<i, L>(B + (size_t)i * g.plane + node, c, a.sp);
+        });
+        mp = p; mu = u; mv = v; mw = w;
+    } else {
+        float s = 0.f, u = 0.f, v = 0.f, w = 0.f;
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = I;
+            s += gv[i];
+            if constexpr (L::ex(i) > 0) u += gv[i];
+            else if constexpr (L::ex(i) < 0) u -= gv[i];
+            if constexpr (L::ey(i) > 0) v += gv[i];
+            else if constexpr (L::ey(i) < 0) v -= gv[i];
+            if constexpr (L::ez(i) > 0) w += gv[i];
+            else if constexpr (L::ez(i) < 0) w -= gv[i];
+        });
+        if constexpr (FORCE && SFRAC) {
+            fx += 2.0 * (double)sf * (double)u;
+            fy += 2.0 * (double)sf * (double)v;
+            fz += 2.0 * (double)sf * (double)w;
+        }
+        // feq_i = w_i (3p - 1.5|u|^2 + e.u (3 + 4.5 e.u)), with 3p = sum g
+        const float base = __builtin_fmaf(-1.5f, __builtin_fmaf(u, u, __builtin_fmaf(v, v, w * w)), s);
+        const float om = a.omega;
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = I;
+            float eu = 0.f;
+            if constexpr (L::ex(i) > 0) eu += u;
+            else if constexpr (L::ex(i) < 0) eu -= u;
+            if constexpr (L::ey(i) > 0) eu += v;
+            else if constexpr (L::ey(i) < 0) eu -= v;
+            if constexpr (L::ez(i) > 0) eu += w;
+            else if constexpr (L::ez(i) < 0) eu -= w;
+            const float feq = (float)L::w(i) * __builtin_fmaf(eu, __builtin_fmaf(4.5f, eu, 3.0f), base);
+            float c = __builtin_fmaf(om, feq - gv[i], gv[i]);
+            if constexpr (SFRAC) c = __builtin_fmaf(sf, gv[L::opp(i)] - c, c);
+            if (live) S::template store<i, L>(B + (size_t)i * g.plane + node, c, a.sp);
+        });
+        mp = (double)s * (1.0 / 3.0); mu = u; mv = v; mw = w;
+    }
+    if constexpr (MACROS) {
+        if (active) {
+            macros[node] = live ? mp : 0.0;
+            macros[g.plane + node] = live ? mu : 0.0;
+            macros[2 * g.plane + node] = live ? mv : 0.0;
+            macros[3 * g.plane + node] = live ? mw : 0.0;
+        }
+    }
+    if constexpr (FORCE) {
+        const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        const int slot = block_slot[bid];
+        if (slot >= 0) {
+            __shared__ double red[3 * 8];
+            block_sum3<8>(fx, fy, fz, red);
+            if (threadIdx.x == 0) {
+                partials[3 * (size_t)slot] = fx;
+                partials[3 * (size_t)slot + 1] = fy;
+                partials[3 * (size_t)slot + 2] = fz;
+            }
+        }
+    }
+}
+
+// second pass of the force reduction: fixed order, one block of 256 lanes
+__global__ void __launch_bounds__(256) k_force_reduce(const double *__restrict__ partials, int nslots,
+                                                      double *__restrict__ hist, int rec, long step) {
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int j = threadIdx.x; j < nslots; j += 256) {
+        a += partials[3 * (size_t)j];
+        b += partials[3 * (size_t)j + 1];
+        c += partials[3 * (size_t)j + 2];
+    }
+    __shared__ double red[3 * 4];
+    block_sum3<4>(a, b, c, red);
+    if (threadIdx.x == 0) {
+        hist[4 * (size_t)rec] = (double)step;
+        hist[4 * (size_t)rec + 1] = a;
+        hist[4 * (size_t)rec + 2] = b;
+        hist[4 * (size_t)rec + 3] = c;
+    }
+}
+
+template <class S>
+__global__ void __launch_bounds__(512) k_decode(const typename S::T *__restrict__ A, double *__restrict__ out,
+                                                const uint8_t *__restrict__ flags, float w, float off, Geo g) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+    if (x >= g.nx) return;
+    const size_t node = (size_t)x + (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
+    const size_t phys = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
+    out[phys] = flags[node] == FL_SOLID ? 0.0 : (double)S::decode(A[node], w, off);
+}
+
+template <class S>
+__global__ void __launch_bounds__(512) k_encode(typename S::T *__restrict__ A, const double *__restrict__ in,
+                                                const uint8_t *__restrict__ flags, float winv, float offw, Geo g) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+    if (x >= g.nx) return;
+    const size_t node = (size_t)x + (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
+    const size_t phys = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
+    A[node] = S::encode(flags[node] == FL_SOLID ? 0.0 : in[phys], winv, offw);
+}
+
+__global__ void __launch_bounds__(512) k_macros_extract(const double *__restrict__ mac, double *__restrict__ out,
+                                                        Geo g) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+    if (x >= g.nx) return;
+    const size_t node = (size_t)x + (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
+    out[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] = mac[node];
+}
+
+// ------------------------------------------------------------------------------------
+// host launchers
+// ------------------------------------------------------------------------------------
+static dim3 grid_of(const Geo &g, const Shape &s) { return dim3(s.gx, g.ny, g.nz); }
+
+template <class F>
+static hipError_t dispatch_store(int prec, F &&f) {
+    switch (prec) {
+    case LBM_FP64: return f(SF64{});
+    case LBM_FP32: return f(SF32{});
+    case LBM_FP16S: return f(SF16{});
+    case LBM_BF16S: return f(SBF16{});
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int Q>
+static hipError_t init_q(int prec, void *A, void *B, double *macros, const uint8_t *flags, const double *uin, const double *xi,
+                         double perturb, uint64_t seed, const FusedArgs &a, hipStream_t s) {
+    const Shape sh = node_shape(a.g);
+    return dispatch_store(prec, [&](auto st) {
+        using S = decltype(st);
+        using T = typename S::T;
+        hipLaunchKernelGGL((k_init<Q, S>), grid_of(a.g, sh), dim3(sh.bx), 0, s, (T *)A, (T *)B, macros, flags, uin,
+                           xi, perturb, seed, a);
+        return hipGetLastError();
+    });
+}
+
+hipError_t plain_init(int Q, int prec, void *A, void *B, double *macros, const uint8_t *flags, const double *uin,
+                      const double *xi, double perturb, uint64_t seed, const FusedArgs &a, hipStream_t s) {
+    return Q == 9 ? init_q<9>(prec, A, B, macros, flags, uin, xi, perturb, seed, a, s)
+                  : init_q<27>(prec, A, B, macros, flags, uin, xi, perturb, seed, a, s);
+}
+
+template <int Q, class S, bool FORCE, bool MACROS>
+static hipError_t step_qs(const void *A, void *B, const uint8_t *flags, const void *sfrac, const double *uin,
+                          const int *block_slot, double *partials, double *macros, const FusedArgs &a,
+                          hipStream_t s) {
+    using T = typename S::T;
+    using C = typename S::C;
+    const Shape sh = node_shape(a.g);
+    if (sfrac)
+        hipLaunchKernelGGL((k_fused<Q, S, FORCE, MACROS, true>), grid_of(a.g, sh), dim3(sh.bx), 0, s,
+                           (const T *)A, (T *)B, flags, (const C *)sfrac, uin, block_slot, partials, macros, a);
+    else
+        hipLaunchKernelGGL((k_fused<Q, S, FORCE, MACROS, false>), grid_of(a.g, sh), dim3(sh.bx), 0, s,
+                           (const T *)A, (T *)B, flags, (const C *)nullptr, uin, block_slot, partials, macros, a);
+    return hipGetLastError();
+}
+
+template <int Q>
+static hipError_t step_q(int prec, const void *A, void *B, const uint8_t *flags, const void *sfrac,
+                         const double *uin, const int *block_slot, double *partials, double *macros, bool force,
+                         bool mac, const FusedArgs &a, hipStream_t s) {
+    return dispatch_store(prec, [&](auto st) {
+        using S = decltype(st);
+        if (force && mac) return step_qs<Q, S, true, true>(A, B, flags, sfrac, uin, block_slot, partials, macros, a, s);
+        if (force) return step_qs<Q, S, true, false>(A, B, flags, sfrac, uin, block_slot, partials, macros, a, s);
+        if (mac) return step_qs<Q, S, false, true>(A, B, flags, sfrac, uin, block_slot, partials, macros, a, s);
+        return step_qs<Q, S, false, false>(A, B, flags, sfrac, uin, block_slot, partials, macros, a, s);
+    });
+}
+
+hipError_t plain_step(int Q, int prec, const void *A, void *B, const uint8_t *flags, const void *sfrac,
+                      const double *uin, const int *block_slot, double *partials, double *macros, bool force,
+                      bool macros_on, const FusedArgs &a, hipStream_t s) {
+    return Q == 9 ? step_q<9>(prec, A, B, flags, sfrac, uin, block_slot, partials, macros, force, macros_on, a, s)
+                  : step_q<27>(prec, A, B, flags, sfrac, uin, block_slot, partials, macros, force, macros_on, a, s);
+}
+
+hipError_t force_reduce(const double *partials, int nslots, double *hist, int rec, long step, hipStream_t s) {
+    hipLaunchKernelGGL(k_force_reduce, dim3(1), dim3(256), 0, s, partials, nslots, hist, rec, step);
+    return hipGetLastError();
+}
+
+template <int Q>
+static double lat_w(int i) {
+    double w = 0;
+    static_for<0, Q>([&](auto I) {
+        if (I == i) w = Lat<Q>::w(I);
+    });
+    return w;
+}
+
+hipError_t plain_decode(int Q, int prec, const void *A, double *out, const uint8_t *flags, int dir,
+                        const FusedArgs &a, hipStream_t s) {
+    const Shape sh = node_shape(a.g);
+    const double w = Q == 9 ? lat_w<9>(dir) : lat_w<27>(dir);
+    return dispatch_store(prec, [&](auto st) {
+        using S = decltype(st);
+        using T = typename S::T;
+        hipLaunchKernelGGL(k_decode<S>, grid_of(a.g, sh), dim3(sh.bx), 0, s,
+                           (const T *)A + (size_t)dir * a.g.plane, out, flags, (float)w, a.sp.off[dir], a.g);
+        return hipGetLastError();
+    });
+}
+
+hipError_t plain_encode(int Q, int prec, void *A, const double *in, const uint8_t *flags, int dir,
+                        const FusedArgs &a, hipStream_t s) {
+    const Shape sh = node_shape(a.g);
+    const double w = Q == 9 ? lat_w<9>(dir) : lat_w<27>(dir);
+    return dispatch_store(prec, [&](auto st) {
+        using S = decltype(st);
+        using T = typename S::T;
+        hipLaunchKernelGGL(k_encode<S>, grid_of(a.g, sh), dim3(sh.bx), 0, s, (T *)A + (size_t)dir * a.g.plane,
+                           in, flags, (float)(1.0 / w), a.sp.offw[dir], a.g);
+        return hipGetLastError();
+    });
+}
+
+hipError_t macros_extract(const double *macros, double *out, int k, const Geo &g, hipStream_t s) {
+    const Shape sh = node_shape(g);
+    hipLaunchKernelGGL(k_macros_extract, grid_of(g, sh), dim3(sh.bx), 0, s, macros + (size_t)k * g.plane, out, g);
+    return hipGetLastError();
+}
+
+}  // namespace lbm

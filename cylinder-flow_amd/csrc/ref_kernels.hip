@@ -1,0 +1,378 @@
+// ref_kernels.hip -- the reference's own boundary rules on gfx950 (FP64 parity path).
+//
+// Data: SoA over the ghost-padded box of the reference (ghost 3 in 3D, 2 in 2D;
+// 3D MPI+KOKKOS+MPIIO/lbm.hpp:20-22,153-163, MPI Kokkos/lbm.hpp:19,79-85):
+//   f[ii*plane + i + lx*(j + ly*k)],  plane = lx*ly*lz.
+// One step is
+//   fills (the reference's six wall/inlet/outlet kernels in their order, in place)
+//   -> usrbc (in place) -> [momentum exchange] -> fused pull + Update + Collision into ft
+//   -> swap f/ft.
+// Fusing stream1/stream2/Update/Collision is exact: the first reference Collision acts on
+// f = feq(macros) and returns f unchanged, so Collision_{n+1} can run right after
+// Update_n inside the pull kernel (3D MPI+KOKKOS+MPIIO/main.cpp:29-37).  Ghost slots are
+// copied f -> ft by the same kernel, so ghost values persist across steps exactly as in
+// the reference's single f array -- including the slots no fill ever refreshes.
+#include "internal.hpp"
+#include "lattice.hpp"
+
+namespace lbm {
+
+__device__ __forceinline__ uint64_t ref_splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+template <int Q, int I>
+__device__ __forceinline__ double ref_feq(double p, double u, double v, double w) {
+    using L = Lat<Q>;
+    double edu = (double)L::ex(I) * u + (double)L::ey(I) * v;
+    double udu = u * u + v * v;
+    if constexpr (L::D == 3) {
+        edu = edu + (double)L::ez(I) * w;
+        udu = udu + w * w;
+    }
+    const double eu2 = edu * edu;
+    return L::w(I) * p * 3.0 + L::w(I) * (3.0 * edu + 4.5 * eu2 - 1.5 * udu);
+}
+
+// u0*4*c*(n-1-c)/(n-1)^2: 3D MPI+KOKKOS+MPIIO/lbm.cpp:239,405 ; MPI Kokkos/lbm.cpp:109,446
+__device__ __forceinline__ double ref_profile(double u0, int c, int n) {
+    return u0 * 4.0 * (double)c * (double)(n - 1 - c) / ((double)(n - 1) * (double)(n - 1));
+}
+
+struct RefBody {
+    int kind;   // LBM_BODY_*
+    double cx, cy, cz, r2;
+    uint64_t seed;
+};
+
+// "random", "initialize"/"initv", "initf" over every node including ghosts
+// (3D MPI+KOKKOS+MPIIO/lbm.cpp:221-254 ; MPI Kokkos/lbm.cpp:92-124)
+template <int Q>
+__global__ void __launch_bounds__(256) k_ref_init(RefGeo r, RefBody b, double *__restrict__ f, double *__restrict__ ft,
+                                                  double *__restrict__ mac, uint8_t *__restrict__ usr,
+                                                  const int32_t *__restrict__ ran_in) {
+    using L = Lat<Q>;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= r.lx) return;
+    const size_t n = (size_t)i + (size_t)r.lx * ((size_t)j + (size_t)r.ly * k);
+    const int ran = ran_in ? ran_in[n] : (int)(ref_splitmix64(b.seed ^ ref_splitmix64(n)) % 100ull);
+    const int x = i - r.g, y = j - r.g, z = (L::D == 3) ? k - r.g : 0;
+    int inside = 0;
+    if (b.kind == LBM_BODY_SPHERE || b.kind == LBM_BODY_ZCYL) {
+        const double dx = (double)x - b.cx, dy = (double)y - b.cy;
+        double rr = dx * dx + dy * dy;
+        if (L::D == 3 && b.kind == LBM_BODY_SPHERE) {
+            const double dz = (double)z - b.cz;
+            rr = rr + dz * dz;
+        }
+        inside = rr <= b.r2;
+    }
+    const int us = inside ? 0 : 1;
+    usr[n] = (uint8_t)us;
+    double u;
+    if constexpr (L::D == 3) u = ref_profile(r.u0, z, r.glz) * (double)us * (1.0 + (double)ran * 0.001);
+    else u = ref_profile(r.u0, y, r.gly) * (double)us * (0.9 + (double)ran * 0.002);
+    mac[n] = 0.0;
+    mac[r.plane + n] = u;
+    mac[2 * r.plane + n] = 0.0;
+    mac[3 * r.plane + n] = 0.0;
+    static_for<0, Q>([&](auto I) {
+        f[(size_t)I * r.plane + n] = ref_feq<Q, I>(0.0, u, 0.0, 0.0);
+        ft[(size_t)I * r.plane + n] = 0.0;
+    });
+}
+
+#define RIDX(r, ii, i, j, k) ((size_t)(ii) * (r).plane + (size_t)((long)(i) + (long)(r).lx * ((long)(j) + (long)(r).ly * (long)(k))))
+
+// 3D fills, one kernel per face: 3D MPI+KOKKOS+MPIIO/lbm.cpp:278-347.
+// face: 0 front (y lo), 1 back (y hi), 2 bottom (z lo), 3 top (z hi), 4 inlet, 5 outlet
+template <int FACE>
+__global__ void __launch_bounds__(256) k_ref_fill3(RefGeo r, double *__restrict__ f) {
+    using L = Lat<27>;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x + r.g - 1;   // first in-face coordinate
+    const int b = blockIdx.y + r.g - 1;                              // second in-face coordinate
+    const int g = r.g;
+    const int le0 = g + r.glx, le1 = g + r.gly, le2 = g + r.glz;
+    if (FACE <= 1) { if (a >= r.lx - g + 1 || b >= r.lz - g + 1) return; }   // (i, k)
+    else if (FACE <= 3) { if (a >= r.lx - g + 1 || b >= r.ly - g + 1) return; }  // (i, j)
+    else { if (a >= r.ly - g + 1 || b >= r.lz - g + 1) return; }   // (j, k)
+    static_for<0, 27>([&](auto I) {
+        constexpr int ii = I;
+        constexpr int ex = L::ex(ii), ey = L::ey(ii), ez = L::ez(ii), bb = L::opp(ii);
+        if constexpr (FACE == 0 && ey > 0)
+            f[RIDX(r, ii, a, g - 1, b)] = f[RIDX(r, bb, a + 2 * ex, g + 1, b + 2 * ez)];
+        if constexpr (FACE == 1 && ey < 0)
+            f[RIDX(r, ii, a, le1, b)] = f[RIDX(r, bb, a + 2 * ex, le1 - 2, b + 2 * ez)];
+        if constexpr (FACE == 2 && ez > 0)
+            f[RIDX(r, ii, a, b, g - 1)] = f[RIDX(r, bb, a + 2 * ex, b + 2 * ey, g + 1)];
+        if constexpr (FACE == 3 && ez < 0)
+            f[RIDX(r, ii, a, b, le2)] = f[RIDX(r, bb, a + 2 * ex, b + 2 * ey, le2 - 2)];
+        if constexpr (FACE == 4 && ex > 0) f[RIDX(r, ii, g - 1, a, b)] = f[RIDX(r, ii, g, a, b)];
+        if constexpr (FACE == 5 && ex < 0)
+            f[RIDX(r, ii, le0, a, b)] = f[RIDX(r, ii, le0 + ex, a + ey, b + ez)];
+    });
+}
+
+// 2D fills: MPI Kokkos/lbm.cpp:315-345 (stream3 inlet, stream3 outlet, stream4 bottom,
+// stream4 top).  Columns i-2 = -1 and i+2 = lx wrap through the flat index, as in the
+// reference's LayoutLeft views.
+template <int FACE>
+__global__ void __launch_bounds__(256) k_ref_fill2(RefGeo r, double *__restrict__ f) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int lx = r.lx, ly = r.ly;
+    if (FACE <= 1) {
+        if (t >= ly - 1) return;
+        const int j = t;
+        if (FACE == 0) {
+            f[RIDX(r, 1, 1, j, 0)] = f[RIDX(r, 1, 2, j, 0)];
+            f[RIDX(r, 5, 1, j, 0)] = f[RIDX(r, 5, 2, j, 0)];
+            f[RIDX(r, 8, 1, j, 0)] = f[RIDX(r, 8, 2, j, 0)];
+        } else {
+            f[RIDX(r, 3, lx - 2, j, 0)] = f[RIDX(r, 3, lx - 3, j, 0)];
+            f[RIDX(r, 7, lx - 2, j, 0)] = f[RIDX(r, 7, lx - 3, j - 1, 0)];
+            f[RIDX(r, 6, lx - 2, j, 0)] = f[RIDX(r, 6, lx - 3, j + 1, 0)];
+        }
+    } else {
+        if (t >= lx - 1) return;
+        const int i = t;
+        if (FACE == 2) {
+            f[RIDX(r, 2, i, 1, 0)] = f[RIDX(r, 4, i, 3, 0)];
+            f[RIDX(r, 5, i, 1, 0)] = f[RIDX(r, 7, i + 2, 3, 0)];
+            f[RIDX(r, 6, i, 1, 0)] = f[RIDX(r, 8, i - 2, 3, 0)];
+        } else {
+            f[RIDX(r, 4, i, ly - 2, 0)] = f[RIDX(r, 2, i, ly - 4, 0)];
+            f[RIDX(r, 7, i, ly - 2, 0)] = f[RIDX(r, 5, i - 2, ly - 4, 0)];
+            f[RIDX(r, 8, i, ly - 2, 0)] = f[RIDX(r, 6, i + 2, ly - 4, 0)];
+        }
+    }
+}
+
+// usrbc: 3D MPI+KOKKOS+MPIIO/lbm.cpp:350-356 ; MPI Kokkos/lbm.cpp:347-353 (in place; the
+// host rejects masks with a solid-fluid-solid run along a lattice link, the only case in
+// which two writes and a read of this kernel touch the same slot)
+template <int Q>
+__global__ void __launch_bounds__(256) k_ref_usrbc(RefGeo r, double *__restrict__ f, const uint8_t *__restrict__ usr) {
+    using L = Lat<Q>;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + r.g, j = blockIdx.y + r.g;
+    const int k = blockIdx.z + (L::D == 3 ? r.g : 0);
+    if (i >= r.lx - r.g) return;
+    const long n = (long)i + (long)r.lx * ((long)j + (long)r.ly * k);
+    if (usr[n] != 0) return;
+    static_for<0, Q>([&](auto I) {
+        constexpr int ii = I;
+        const long nb = n + L::ex(ii) + (long)r.lx * (L::ey(ii) + (long)r.ly * L::ez(ii));
+        if (usr[nb] == 1) {
+            const long n2 = n + 2 * (L::ex(ii) + (long)r.lx * (L::ey(ii) + (long)r.ly * L::ez(ii)));
+            f[(size_t)ii * r.plane + n] = f[(size_t)L::opp(ii) * r.plane + n2];
+        }
+    });
+}
+
+// momentum exchange after usrbc: sum over fluid X, solid X+e of (f_i(X) + f_bb(i)(X+e)) e
+// (MPI Kokkos/lbm.cpp:364-392), per-block partials then a fixed-order second pass
+template <int Q>
+__global__ void __launch_bounds__(256) k_ref_mem(RefGeo r, const double *__restrict__ f, const uint8_t *__restrict__ usr,
+                                                 double *__restrict__ partials) {
+    using L = Lat<Q>;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + r.g, j = blockIdx.y + r.g;
+    const int k = blockIdx.z + (L::D == 3 ? r.g : 0);
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (i < r.lx - r.g) {
+        const long n = (long)i + (long)r.lx * ((long)j + (long)r.ly * k);
+        if (usr[n] == 1) {
+            static_for<0, Q>([&](auto I) {
+                constexpr int ii = I;
+                const long nb = n + L::ex(ii) + (long)r.lx * (L::ey(ii) + (long)r.ly * L::ez(ii));
+                if (usr[nb] == 0) {
+                    const double m = f[(size_t)ii * r.plane + n] + f[(size_t)L::opp(ii) * r.plane + nb];
+                    fx += m * (double)L::ex(ii);
+                    fy += m * (double)L::ey(ii);
+                    fz += m * (double)L::ez(ii);
+                }
+            });
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        fx += __shfl_xor(fx, off);
+        fy += __shfl_xor(fy, off);
+        fz += __shfl_xor(fz, off);
+    }
+    __shared__ double red[3 * 4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[3 * wv] = fx;
+        red[3 * wv + 1] = fy;
+        red[3 * wv + 2] = fz;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < 4; q++) {
+            red[0] += red[3 * q];
+            red[1] += red[3 * q + 1];
+            red[2] += red[3 * q + 2];
+        }
+        const size_t blk = blockIdx.x + (size_t)gridDim.x * (blockIdx.y + (size_t)gridDim.y * blockIdx.z);
+        partials[3 * blk] = red[0];
+        partials[3 * blk + 1] = red[1];
+        partials[3 * blk + 2] = red[2];
+    }
+}
+
+// stream1 + stream2 + Update (with its inlet/outlet overrides) + the next Collision:
+// 3D MPI+KOKKOS+MPIIO/lbm.cpp:257-272,359-417 ; MPI Kokkos/lbm.cpp:127-139,408-457
+template <int Q>
+__global__ void __launch_bounds__(256) k_ref_fused(RefGeo r, const double *__restrict__ f, double *__restrict__ ft,
+                                                   double *__restrict__ mac, const uint8_t *__restrict__ usr,
+                                                   const double *__restrict__ prof) {
+    using L = Lat<Q>;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= r.lx) return;
+    const size_t n = (size_t)i + (size_t)r.lx * ((size_t)j + (size_t)r.ly * k);
+    const int g = r.g;
+    const bool interior = i >= g && i < r.lx - g && j >= g && j < r.ly - g &&
+                          (L::D == 2 || (k >= g && k < r.lz - g));
+    if (!interior) {
+        static_for<0, Q>([&](auto I) { ft[(size_t)I * r.plane + n] = f[(size_t)I * r.plane + n]; });
+        return;
+    }
+    double fv[Q];
+    double p = 0.0, u = 0.0, v = 0.0, w = 0.0;
+    static_for<0, Q>([&](auto I) {
+        constexpr int ii = I;
+        const long src = (long)n - (L::ex(ii) + (long)r.lx * (L::ey(ii) + (long)r.ly * L::ez(ii)));
+        fv[ii] = f[(size_t)ii * r.plane + src];
+        p = p + fv[ii] / 3.0;
+        if constexpr (L::ex(ii) > 0) u = u + fv[ii];
+        else if constexpr (L::ex(ii) < 0) u = u - fv[ii];
+        if constexpr (L::ey(ii) > 0) v = v + fv[ii];
+        else if constexpr (L::ey(ii) < 0) v = v - fv[ii];
+        if constexpr (L::ez(ii) > 0) w = w + fv[ii];
+        else if constexpr (L::ez(ii) < 0) w = w - fv[ii];
+    });
+    const double us = (double)usr[n];
+    u = u * us;
+    v = v * us;
+    w = w * us;
+    p = p * us;
+    if (i == g) {
+        u = prof[L::D == 3 ? k - g : j - g];
+        v = 0.0;
+        w = 0.0;
+    }
+    if (i == r.lx - g - 1) p = 0.0;
+    mac[n] = p;
+    mac[r.plane + n] = u;
+    mac[2 * r.plane + n] = v;
+    mac[3 * r.plane + n] = w;
+    static_for<0, Q>([&](auto I) {
+        constexpr int ii = I;
+        const double feq = ref_feq<Q, ii>(p, u, v, w);
+        ft[(size_t)ii * r.plane + n] = fv[ii] - (fv[ii] - feq) / (r.tau0 + 0.5);
+    });
+}
+
+__global__ void __launch_bounds__(256) k_ref_reduce(const double *__restrict__ partials, int n, double *__restrict__ hist,
+                                                    int rec, long step) {
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int j = threadIdx.x; j < n; j += 256) {
+        a += partials[3 * (size_t)j];
+        b += partials[3 * (size_t)j + 1];
+        c += partials[3 * (size_t)j + 2];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+        c += __shfl_xor(c, off);
+    }
+    __shared__ double red[12];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[3 * wv] = a;
+        red[3 * wv + 1] = b;
+        red[3 * wv + 2] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < 4; q++) {
+            red[0] += red[3 * q];
+            red[1] += red[3 * q + 1];
+            red[2] += red[3 * q + 2];
+        }
+        hist[4 * (size_t)rec] = (double)step;
+        hist[4 * (size_t)rec + 1] = red[0];
+        hist[4 * (size_t)rec + 2] = red[1];
+        hist[4 * (size_t)rec + 3] = red[2];
+    }
+}
+
+// ------------------------------------------------------------------------------------
+static dim3 interior_grid(const RefGeo &r) {
+    return dim3((r.glx + 255) / 256, r.gly, r.D == 3 ? r.glz : 1);
+}
+
+int ref_mem_blocks(const RefGeo &r) {
+    const dim3 g = interior_grid(r);
+    return (int)(g.x * g.y * g.z);
+}
+
+hipError_t ref_init(const RefGeo &r, double *f, double *ft, double *mac, uint8_t *usr, const int32_t *ran,
+                    int body, double cx, double cy, double cz, double r2, uint64_t seed, hipStream_t s) {
+    RefBody b;
+    b.kind = body;
+    b.cx = cx;
+    b.cy = cy;
+    b.cz = cz;
+    b.r2 = r2;
+    b.seed = seed;
+    const dim3 grid((r.lx + 255) / 256, r.ly, r.lz);
+    if (r.Q == 9) hipLaunchKernelGGL(k_ref_init<9>, grid, dim3(256), 0, s, r, b, f, ft, mac, usr, ran);
+    else hipLaunchKernelGGL(k_ref_init<27>, grid, dim3(256), 0, s, r, b, f, ft, mac, usr, ran);
+    return hipGetLastError();
+}
+
+hipError_t ref_fills(const RefGeo &r, double *f, hipStream_t s) {
+    if (r.D == 3) {
+        const int nI = r.lx - 2 * (r.g - 1), nJ = r.ly - 2 * (r.g - 1), nK = r.lz - 2 * (r.g - 1);
+        hipLaunchKernelGGL(k_ref_fill3<0>, dim3((nI + 255) / 256, nK), dim3(256), 0, s, r, f);
+        hipLaunchKernelGGL(k_ref_fill3<1>, dim3((nI + 255) / 256, nK), dim3(256), 0, s, r, f);
+        hipLaunchKernelGGL(k_ref_fill3<2>, dim3((nI + 255) / 256, nJ), dim3(256), 0, s, r, f);
+        hipLaunchKernelGGL(k_ref_fill3<3>, dim3((nI + 255) / 256, nJ), dim3(256), 0, s, r, f);
+        hipLaunchKernelGGL(k_ref_fill3<4>, dim3((nJ + 255) / 256, nK), dim3(256), 0, s, r, f);
+        hipLaunchKernelGGL(k_ref_fill3<5>, dim3((nJ + 255) / 256, nK), dim3(256), 0, s, r, f);
+    } else {
+        hipLaunchKernelGGL(k_ref_fill2<0>, dim3((r.ly + 255) / 256), dim3(256), 0, s, r, f);
+        hipLaunchKernelGGL(k_ref_fill2<1>, dim3((r.ly + 255) / 256), dim3(256), 0, s, r, f);
+        hipLaunchKernelGGL(k_ref_fill2<2>, dim3((r.lx + 255) / 256), dim3(256), 0, s, r, f);
+        hipLaunchKernelGGL(k_ref_fill2<3>, dim3((r.lx + 255) / 256), dim3(256), 0, s, r, f);
+    }
+    return hipGetLastError();
+}
+
+hipError_t ref_usrbc(const RefGeo &r, double *f, const uint8_t *usr, hipStream_t s) {
+    if (r.Q == 9) hipLaunchKernelGGL(k_ref_usrbc<9>, interior_grid(r), dim3(256), 0, s, r, f, usr);
+    else hipLaunchKernelGGL(k_ref_usrbc<27>, interior_grid(r), dim3(256), 0, s, r, f, usr);
+    return hipGetLastError();
+}
+
+hipError_t ref_mem(const RefGeo &r, const double *f, const uint8_t *usr, double *partials, int nblk, double *hist,
+                   int rec, long step, hipStream_t s) {
+    if (r.Q == 9) hipLaunchKernelGGL(k_ref_mem<9>, interior_grid(r), dim3(256), 0, s, r, f, usr, partials);
+    else hipLaunchKernelGGL(k_ref_mem<27>, interior_grid(r), dim3(256), 0, s, r, f, usr, partials);
+    hipLaunchKernelGGL(k_ref_reduce, dim3(1), dim3(256), 0, s, partials, nblk, hist, rec, step);
+    return hipGetLastError();
+}
+
+hipError_t ref_fused(const RefGeo &r, const double *f, double *ft, double *mac, const uint8_t *usr,
+                     const double *prof, hipStream_t s) {
+    const dim3 grid((r.lx + 255) / 256, r.ly, r.lz);
+    if (r.Q == 9) hipLaunchKernelGGL(k_ref_fused<9>, grid, dim3(256), 0, s, r, f, ft, mac, usr, prof);
+    else hipLaunchKernelGGL(k_ref_fused<27>, grid, dim3(256), 0, s, r, f, ft, mac, usr, prof);
+    return hipGetLastError();
+}
+
+}  // namespace lbm

@@ -1,0 +1,181 @@
+"""GPU (HIP) vs CPU oracle parity on seeded small inputs, through the C ABI.
+
+FP64 runs are compared at round-off level (the kernels follow the oracle's rounding order,
+both built without FMA contraction); forces differ only by summation order.  FP32 and the
+16-bit storage formats are compared to the FP64 oracle with stated error budgets.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def plain_pair(pkg, orc, Q, nx, ny, nz=1, *, precision, body=0, center=(0.0, 0.0, 0.0), r2=0.0,
+               periodic=(False, False, False), inlet=0, coef=(), perturb=1e-3, tau0=0.06, u0=0.05,
+               force_interval=1, store_uref=0.0):
+    L = pkg.lbm
+    g = L.make_config(Q, nx, ny, nz, tau0=tau0, u0=u0, precision=precision, bc_mode=L.BC_PLAIN,
+                      periodic=periodic, body=body, center=center, r2=r2, inlet=inlet, coef=coef,
+                      perturb=perturb, force_interval=force_interval, store_uref=store_uref)
+    o = orc.make_cfg(Q, nx, ny, nz, tau0=tau0, u0=u0, body=body, center=center, r2=r2, inlet=inlet,
+                     coef=coef, perturb=perturb, periodic=periodic, force_interval=force_interval)
+    return g, o
+
+
+PLAIN_CASES = {
+    # name: (Q, nx, ny, nz, kwargs, steps)
+    "d2q9_disk_walls_uniform": (9, 72, 40, 1, dict(body=1, center=(20.0, 19.5, 0.0), r2=25.0), 80),
+    "d2q9_channel_parabolic": (9, 70, 33, 1, dict(inlet=1), 60),
+    "d2q9_periodic_disk": (9, 48, 36, 1, dict(body=1, center=(24.0, 17.0, 0.0), r2=20.0,
+                                              periodic=(True, True, False)), 60),
+    "d3q27_sphere_walls_parabolic": (27, 36, 20, 18, dict(body=1, center=(10.0, 9.5, 8.7), r2=16.0, inlet=1), 40),
+    "d3q27_zcyl_legendre": (27, 40, 22, 14, dict(body=2, center=(12.0, 11.0, 0.0), r2=12.0, inlet=2,
+                                                  coef=(0.04, 0.005, -0.004, 0.003, 0.002, -0.001, 0.001, 0.0005)), 40),
+    "d3q27_periodic_yz_sphere": (27, 32, 18, 16, dict(body=1, center=(12.0, 9.0, 8.0), r2=10.0,
+                                                       periodic=(False, True, True)), 40),
+    "d3q27_fully_periodic": (27, 24, 20, 18, dict(body=1, center=(12.0, 10.0, 9.0), r2=9.0,
+                                                   periodic=(True, True, True)), 30),
+}
+
+
+def run_pair(pkg, orc, name, precision, sfrac=False):
+    Q, nx, ny, nz, kw, steps = PLAIN_CASES[name]
+    g, o = plain_pair(pkg, orc, Q, nx, ny, nz, precision=precision, **kw)
+    s = None
+    if sfrac:
+        shape = (nz, ny, nx)
+        rng = np.random.default_rng(7)
+        s = np.zeros(shape)
+        idx = tuple(rng.integers(0, n, 12) for n in shape)
+        s[idx] = rng.uniform(0.0, 1.0, 12)
+        if precision != pkg.lbm.FP64:
+            s = s.astype(np.float32).astype(np.float64)
+    gs = pkg.Solver(g)
+    if s is not None:
+        gs.set_solid_fraction(s)
+    gs.initialize()
+    os_ = orc.PlainState(o, sfrac=s)
+    print(f"[parity] {name} prec={precision} sfrac={sfrac} steps={steps}", flush=True)
+    gs.step(steps)
+    os_.step(steps)
+    return gs, os_
+
+
+@pytest.mark.parametrize("name", sorted(PLAIN_CASES))
+def test_plain_fp64_matches_oracle(pkg, orc, name):
+    gs, os_ = run_pair(pkg, orc, name, pkg.lbm.FP64, sfrac=name.startswith("d2q9_periodic") or "periodic_yz" in name)
+    fg, fo = gs.populations(), os_.populations()
+    assert rel_l2(fg, fo) <= 1e-13, (rel_l2(fg, fo), np.abs(fg - fo).max())
+    for a, b in zip(gs.fields(), os_.fields()):
+        assert np.abs(a - b).max() <= 1e-13 * max(1.0, np.abs(b).max())
+    Fg, Fo = gs.forces(), os_.forces()
+    assert Fg.shape == Fo.shape and np.array_equal(Fg[:, 0], Fo[:, 0])
+    scale = max(np.abs(Fo[:, 1:]).max(), 1e-300)
+    assert np.abs(Fg[:, 1:] - Fo[:, 1:]).max() <= 1e-11 * scale
+
+
+@pytest.mark.parametrize("name", sorted(PLAIN_CASES))
+def test_plain_fp32_within_budget(pkg, orc, name):
+    gs, os_ = run_pair(pkg, orc, name, pkg.lbm.FP32)
+    # FP32 collision and storage vs the FP64 oracle: error ~ steps * 2^-24 relative
+    p_g, u_g, v_g, w_g = gs.fields()
+    p_o, u_o, v_o, w_o = os_.fields()
+    assert rel_l2(u_g, u_o) <= 2e-5, rel_l2(u_g, u_o)
+    assert rel_l2(gs.populations(), os_.populations()) <= 2e-5
+
+
+@pytest.mark.parametrize("precision,budget", [("fp16s", 3e-3), ("bf16s", 3e-2)])
+@pytest.mark.parametrize("name", ["d2q9_disk_walls_uniform", "d3q27_sphere_walls_parabolic"])
+def test_plain_16bit_within_budget(pkg, orc, name, precision, budget):
+    gs, os_ = run_pair(pkg, orc, name, pkg.lbm.PRECISIONS[precision])
+    u_g = gs.fields()[1]
+    u_o = os_.fields()[1]
+    err = rel_l2(u_g, u_o)
+    print(f"[parity] {name} {precision} velocity rel-L2 {err:.3e}")
+    assert err <= budget
+
+
+REF_CASES = {
+    # reference geometry formula (tiny body) and enlarged bodies; (Q, nx, ny, nz, body kwargs, u0, tau0, steps)
+    "ref2d_default_disk": (9, 64, 32, 1, None, 0.1, 0.036, 60),
+    "ref2d_disk": (9, 80, 36, 1, dict(center=(22.0, 17.5, 0.0), r2=20.0), 0.1, 0.036, 60),
+    "ref3d_default_sphere": (27, 40, 16, 14, None, 0.1, 0.024, 30),
+    "ref3d_sphere": (27, 36, 18, 16, dict(center=(10.0, 8.0, 7.0), r2=9.0), 0.1, 0.024, 30),
+}
+
+
+@pytest.mark.parametrize("name", sorted(REF_CASES))
+def test_reference_mode_matches_oracle(pkg, orc, name):
+    L = pkg.lbm
+    Q, nx, ny, nz, body, u0, tau0, steps = REF_CASES[name]
+    if body is None:  # the reference's own body formula
+        if Q == 27:
+            c = float((nx - 1) // 8)
+            body = dict(center=(c, c, c), r2=((nx - 1) / 64.0) ** 2)
+        else:
+            r = 0.03125 * (nx - 1) / 2.0
+            body = dict(center=(0.5 * (nx - 1), 0.5 * (ny - 1), 0.0), r2=r * r)
+    g = L.make_config(Q, nx, ny, nz, tau0=tau0, u0=u0, precision=L.FP64, bc_mode=L.BC_REFERENCE,
+                      body=L.BODY_SPHERE, force_interval=1, **body)
+    o = orc.make_cfg(Q, nx, ny, nz, tau0=tau0, u0=u0, body=orc.BODY_SPHERE, force_interval=1, **body)
+    print(f"[parity] {name} reference mode, {steps} steps", flush=True)
+    gs = pkg.Solver(g).initialize()
+    rs = orc.RefState(o)
+    gs.step(steps)
+    rs.step(steps)
+    for a, b in zip(gs.fields(), rs.fields()):
+        assert rel_l2(a, b) <= 1e-12 or np.abs(b).max() == 0.0
+        assert np.abs(a - b).max() <= 1e-13 * max(1.0, np.abs(b).max())
+    assert rel_l2(gs.populations(), rs.populations()) <= 1e-13
+    Fg, Fo = gs.forces(), rs.forces()
+    assert Fg.shape == Fo.shape
+    assert np.abs(Fg[:, 1:] - Fo[:, 1:]).max() <= 1e-11 * max(np.abs(Fo[:, 1:]).max(), 1e-300)
+
+
+def test_reference_mode_injected_ran(pkg, orc):
+    """An injected ran field (in place of the Kokkos pool draw) reaches both sides identically."""
+    L = pkg.lbm
+    nx, ny = 50, 24
+    g = L.make_config(9, nx, ny, tau0=0.036, u0=0.1, precision=L.FP64, bc_mode=L.BC_REFERENCE,
+                      body=L.BODY_SPHERE, center=(14.0, 11.5, 0.0), r2=12.0, force_interval=5)
+    o = orc.make_cfg(9, nx, ny, tau0=0.036, u0=0.1, body=orc.BODY_SPHERE, center=(14.0, 11.5, 0.0), r2=12.0,
+                     force_interval=5)
+    ran = np.random.default_rng(3).integers(0, 100, (ny + 4) * (nx + 4)).astype(np.int32)
+    gs = pkg.Solver(g)
+    gs.set_ran(ran)
+    gs.initialize()
+    rs = orc.RefState(o, ran=ran)
+    gs.step(40)
+    rs.step(40)
+    assert rel_l2(gs.fields()[1], rs.fields()[1]) <= 1e-13
+
+
+@pytest.mark.parametrize("Q,shape,prec,tol", [(9, (1, 20, 40), "fp64", 1e-9), (27, (10, 10, 16), "fp64", 1e-9),
+                                              (9, (1, 20, 40), "fp32", 2e-3)])
+def test_adjoint_matches_cpu_adjoint(pkg, orc, Q, shape, prec, tol):
+    L = pkg.lbm
+    nz, ny, nx = shape
+    rng = np.random.default_rng(11)
+    zz, yy, xx = np.mgrid[0:nz, 0:ny, 0:nx]
+    s = np.exp(-((xx - nx / 3) ** 2 + (yy - ny / 2) ** 2 + (zz - nz / 2) ** 2 * (Q == 27)) / 6.0)
+    s = np.clip(s + 0.01 * rng.standard_normal(s.shape), 0.0, 1.0)
+    if prec == "fp32":
+        s = s.astype(np.float32).astype(np.float64)
+    mask = np.zeros(shape, np.uint8)
+    mask[nz // 2, ny // 4, (2 * nx) // 3] = 1
+    n0, n1, K = 8, 30, 7
+    g = L.make_config(Q, nx, ny, nz, tau0=0.1, u0=0.05, precision=L.PRECISIONS[prec], body=L.BODY_MASK,
+                      perturb=1e-3, inlet=L.INLET_PARABOLIC)
+    o = orc.make_cfg(Q, nx, ny, nz, tau0=0.1, u0=0.05, body=orc.BODY_MASK, perturb=1e-3, inlet=orc.INLET_PARABOLIC)
+    print(f"[parity] adjoint Q={Q} {shape} {prec}", flush=True)
+    gs = pkg.Solver(g)
+    gs.set_mask(mask)
+    gs.set_solid_fraction(s)
+    gs.initialize()
+    Jg, dsg, dug, tf, ta = gs.adjoint(n0, n1, 1.0, K)
+    Jo, dso, duo = orc.adjoint(o, mask=mask, sfrac=s, n0=n0, n1=n1, scale=1.0, K=K)
+    assert abs(Jg - Jo) <= tol * abs(Jo)
+    assert rel_l2(dsg, dso) <= tol, rel_l2(dsg, dso)
+    assert rel_l2(dug, duo) <= tol, rel_l2(dug, duo)
