@@ -113,8 +113,23 @@ def adjoint_ratio(pkg, steps=500):
             "forward_ms": round(tf, 3), "adjoint_ms": round(ta, 3), "ratio": round(ta / tf, 3)}
 
 
+def copy_calibration(n_bytes=1 << 30, reps=3):
+    """A plain device-to-device copy of known size (read n_bytes + write n_bytes): its
+    FETCH_SIZE / WRITE_SIZE counters give the scale factor from counter units to bytes."""
+    import torch
+    a = torch.empty(n_bytes // 4, dtype=torch.float32, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return {"bytes_read": n_bytes, "bytes_written": n_bytes, "reps": reps}
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 1 GiB device copy (reference for profiler byte counters)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
@@ -170,6 +185,9 @@ def main():
     if not a.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+    if a.calibrate_copy:
+        log("copy calibration")
+        out["copy_calibration"] = copy_calibration()
     print(json.dumps(out), flush=True)
 
 

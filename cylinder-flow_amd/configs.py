@@ -39,7 +39,7 @@ def cfg0_cylinder2d(precision=L.FP64, **kw) -> Workload:
     u, D, Re = 0.05, 20.0, 100.0
     c = L.make_config(9, 400, 100, tau0=3 * u * D / Re, u0=u, precision=precision, body=L.BODY_SPHERE,
                       center=(80.0, 50.0, 0.0), r2=(D / 2) ** 2, seed=SEED, perturb=1e-3,
-                      force_interval=10, **kw)
+                      **{"force_interval": 10, **kw})
     return Workload("cfg0_d2q9_cylinder_400x100_re100", c, 20000, D, D, u)
 
 
@@ -48,7 +48,7 @@ def cfg1_sphere3d(precision=L.FP32, **kw) -> Workload:
     u, D, Re = 0.05, 24.0, 200.0
     c = L.make_config(27, 256, 128, 128, tau0=3 * u * D / Re, u0=u, precision=precision, body=L.BODY_SPHERE,
                       center=(64.0, 64.0, 64.0), r2=(D / 2) ** 2, seed=SEED, perturb=1e-3,
-                      force_interval=10, **kw)
+                      **{"force_interval": 10, **kw})
     return Workload("cfg1_d3q27_sphere_256x128x128_re200", c, 10000, D, math.pi * D * D / 4, u)
 
 
@@ -57,7 +57,7 @@ def cfg2_zcyl3d(precision=L.FP16S, **kw) -> Workload:
     u, D, Re = 0.05, 40.0, 300.0
     c = L.make_config(27, 512, 256, 256, tau0=3 * u * D / Re, u0=u, precision=precision, body=L.BODY_ZCYL,
                       center=(128.0, 128.0, 0.0), r2=(D / 2) ** 2, seed=SEED, perturb=1e-3,
-                      force_interval=10, **kw)
+                      **{"force_interval": 10, **kw})
     return Workload("cfg2_d3q27_zcylinder_512x256x256_re300", c, 20000, D, D * 256, u)
 
 

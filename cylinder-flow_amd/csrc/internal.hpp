@@ -65,8 +65,9 @@ hipError_t ref_fills(const RefGeo &r, double *f, hipStream_t s);
 hipError_t ref_usrbc(const RefGeo &r, double *f, const uint8_t *usr, hipStream_t s);
 hipError_t ref_mem(const RefGeo &r, const double *f, const uint8_t *usr, double *partials, int nblk,
                    double *hist, int rec, long step, hipStream_t s);
+// fpre (may be null): receives the pulled populations, i.e. the reference's f after Update
 hipError_t ref_fused(const RefGeo &r, const double *f, double *ft, double *mac, const uint8_t *usr,
-                     const double *prof, hipStream_t s);
+                     const double *prof, double *fpre, hipStream_t s);
 int ref_mem_blocks(const RefGeo &r);
 
 // ---- adjoint (plain mode) ----
@@ -120,7 +121,7 @@ struct lbm_ctx {
 
     // reference-mode device state
     lbm::RefGeo rg{};
-    double *rf = nullptr, *rft = nullptr, *rmac = nullptr, *rprof = nullptr;
+    double *rf = nullptr, *rft = nullptr, *rmac = nullptr, *rprof = nullptr, *rfpre = nullptr;
     uint8_t *rusr = nullptr;
 
     bool initialized = false;

@@ -89,7 +89,7 @@ extern "C" const char *lbm_last_error(const lbm_ctx *ctx) { return ctx ? ctx->er
 
 static void free_all(lbm_ctx *c) {
     void *ptrs[] = {c->A, c->B, c->flags, c->dsfrac, c->duin, c->macros, c->block_slot, c->partials, c->hist,
-                    c->rf, c->rft, c->rmac, c->rprof, c->rusr};
+                    c->rf, c->rft, c->rmac, c->rprof, c->rusr, c->rfpre};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c->A = c->B = nullptr;
@@ -97,7 +97,7 @@ static void free_all(lbm_ctx *c) {
     c->dsfrac = nullptr;
     c->duin = c->macros = c->partials = c->hist = nullptr;
     c->block_slot = nullptr;
-    c->rf = c->rft = c->rmac = c->rprof = nullptr;
+    c->rf = c->rft = c->rmac = c->rprof = c->rfpre = nullptr;
     c->rusr = nullptr;
 }
 
@@ -414,6 +414,7 @@ static int init_ref(lbm_ctx *c) {
     HIP_TRY(c, hipMalloc(&c->rft, sizeof(double) * Q * r.plane));
     HIP_TRY(c, hipMalloc(&c->rmac, sizeof(double) * 4 * r.plane));
     HIP_TRY(c, hipMalloc(&c->rusr, r.plane));
+    HIP_TRY(c, hipMalloc(&c->rfpre, sizeof(double) * Q * r.plane));
     const int nprof = c->D == 3 ? r.glz : r.gly;
     std::vector<double> prof(nprof);
     for (int k = 0; k < nprof; k++)   // u0*4*c*(n-1-c)/(n-1)^2, lbm.cpp:405 / MPI Kokkos/lbm.cpp:446
@@ -427,6 +428,8 @@ static int init_ref(lbm_ctx *c) {
     }
     HIP_TRY(c, ref_init(r, c->rf, c->rft, c->rmac, c->rusr, dran, c->cfg.body, c->cfg.cx, c->cfg.cy, c->cfg.cz,
                         c->cfg.r2, c->cfg.seed, c->stream));
+    // before the first step the reference's f is the initial equilibrium
+    HIP_TRY(c, hipMemcpyAsync(c->rfpre, c->rf, sizeof(double) * Q * r.plane, hipMemcpyDeviceToDevice, c->stream));
     std::vector<uint8_t> usr(r.plane);
     HIP_TRY(c, hipMemcpyAsync(usr.data(), c->rusr, r.plane, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -482,7 +485,8 @@ static int step_n(lbm_ctx *c, int n) {
             HIP_TRY(c, ref_fills(c->rg, c->rf, c->stream));
             HIP_TRY(c, ref_usrbc(c->rg, c->rf, c->rusr, c->stream));
             if (force) HIP_TRY(c, ref_mem(c->rg, c->rf, c->rusr, c->partials, c->nslots, c->hist, c->nrec++, c->step, c->stream));
-            HIP_TRY(c, ref_fused(c->rg, c->rf, c->rft, c->rmac, c->rusr, c->rprof, c->stream));
+            HIP_TRY(c, ref_fused(c->rg, c->rf, c->rft, c->rmac, c->rusr, c->rprof, k == n - 1 ? c->rfpre : nullptr,
+                                 c->stream));
             std::swap(c->rf, c->rft);
         }
         return LBM_OK;
@@ -593,7 +597,7 @@ extern "C" int lbm_get_populations(lbm_ctx *c, double *f) {
     if (c->cfg.bc_mode == LBM_BC_REFERENCE) {
         std::vector<double> pad(c->rg.plane);
         for (int i = 0; i < c->Q; i++) {
-            HIP_TRY(c, hipMemcpyAsync(pad.data(), c->rf + (size_t)i * c->rg.plane, sizeof(double) * c->rg.plane,
+            HIP_TRY(c, hipMemcpyAsync(pad.data(), c->rfpre + (size_t)i * c->rg.plane, sizeof(double) * c->rg.plane,
                                       hipMemcpyDeviceToHost, c->stream));
             HIP_TRY(c, hipStreamSynchronize(c->stream));
             ref_strip(c, pad, f + (size_t)i * c->N);

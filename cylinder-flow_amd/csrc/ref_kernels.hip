@@ -227,7 +227,7 @@ __global__ void __launch_bounds__(256) k_ref_mem(RefGeo r, const double *__restr
 template <int Q>
 __global__ void __launch_bounds__(256) k_ref_fused(RefGeo r, const double *__restrict__ f, double *__restrict__ ft,
                                                    double *__restrict__ mac, const uint8_t *__restrict__ usr,
-                                                   const double *__restrict__ prof) {
+                                                   const double *__restrict__ prof, double *__restrict__ fpre) {
     using L = Lat<Q>;
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
     if (i >= r.lx) return;
@@ -245,6 +245,7 @@ __global__ void __launch_bounds__(256) k_ref_fused(RefGeo r, const double *__res
         constexpr int ii = I;
         const long src = (long)n - (L::ex(ii) + (long)r.lx * (L::ey(ii) + (long)r.ly * L::ez(ii)));
         fv[ii] = f[(size_t)ii * r.plane + src];
+        if (fpre) fpre[(size_t)ii * r.plane + n] = fv[ii];
         p = p + fv[ii] / 3.0;
         if constexpr (L::ex(ii) > 0) u = u + fv[ii];
         else if constexpr (L::ex(ii) < 0) u = u - fv[ii];
@@ -368,10 +369,10 @@ hipError_t ref_mem(const RefGeo &r, const double *f, const uint8_t *usr, double 
 }
 
 hipError_t ref_fused(const RefGeo &r, const double *f, double *ft, double *mac, const uint8_t *usr,
-                     const double *prof, hipStream_t s) {
+                     const double *prof, double *fpre, hipStream_t s) {
     const dim3 grid((r.lx + 255) / 256, r.ly, r.lz);
-    if (r.Q == 9) hipLaunchKernelGGL(k_ref_fused<9>, grid, dim3(256), 0, s, r, f, ft, mac, usr, prof);
-    else hipLaunchKernelGGL(k_ref_fused<27>, grid, dim3(256), 0, s, r, f, ft, mac, usr, prof);
+    if (r.Q == 9) hipLaunchKernelGGL(k_ref_fused<9>, grid, dim3(256), 0, s, r, f, ft, mac, usr, prof, fpre);
+    else hipLaunchKernelGGL(k_ref_fused<27>, grid, dim3(256), 0, s, r, f, ft, mac, usr, prof, fpre);
     return hipGetLastError();
 }
 

@@ -124,7 +124,9 @@ int lbm_sync(lbm_ctx *ctx);
 /* macroscopic fields after the last step (the arrays Update writes:
  * p, ua, va, wa -- 3D MPI+KOKKOS+MPIIO/lbm.cpp:384-415); any pointer may be NULL */
 int lbm_get_fields(lbm_ctx *ctx, double *p, double *u, double *v, double *w);
-/* populations (decoded to FP64), Q*N doubles */
+/* populations (decoded to FP64), Q*N doubles.  Reference mode: the reference's f after
+ * Update (streamed, not yet collided: 3D MPI+KOKKOS+MPIIO/lbm.cpp:359-370).  Plain mode:
+ * the stored state (post-collision), which lbm_set_populations replaces (restart). */
 int lbm_get_populations(lbm_ctx *ctx, double *f);
 int lbm_set_populations(lbm_ctx *ctx, const double *f);
 /* momentum-exchange force records (step, Fx, Fy, Fz): MPI Kokkos/lbm.cpp:356-407.

@@ -116,6 +116,14 @@ static double feq_ref(const lattice *L, int ii, double p, double u, double v, do
     return L->t[ii] * p * 3.0 + L->t[ii] * (3.0 * edu + 4.5 * eu2 - 1.5 * udu);
 }
 
+int orc_feq(int Q, double p, double u, double v, double w, double *out)
+{
+    lattice L;
+    if (make_lattice(Q, &L)) return -1;
+    for (int i = 0; i < Q; i++) out[i] = feq_ref(&L, i, p, u, v, w);
+    return 0;
+}
+
 static void push_force(double **rec, int *n, int *cap, long step, const double F[3])
 {
     if (*n == *cap) {

@@ -59,6 +59,8 @@ int      orc_ran100(uint64_t seed, uint64_t index);  /* integer in [0, 100) */
 
 /* ---- lattice tables in the reference order ---- */
 int orc_lattice(int Q, int *e /* Q*3 */, double *w /* Q */, int *bb /* Q */);
+/* the reference equilibrium for one node (Q values) */
+int orc_feq(int Q, double p, double u, double v, double w, double *out);
 
 /* ---- reference mode ---- */
 typedef struct orc_ref orc_ref;

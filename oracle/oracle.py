@@ -53,6 +53,7 @@ def lib():
         L.orc_ran100.restype = C.c_int
         L.orc_ran100.argtypes = [C.c_uint64, C.c_uint64]
         L.orc_lattice.argtypes = [C.c_int, P, P, P]
+        L.orc_feq.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, P]
         L.orc_ref_create.restype = P
         L.orc_ref_create.argtypes = [C.POINTER(OrcCfg), P]
         L.orc_ref_destroy.argtypes = [P]
@@ -119,6 +120,12 @@ def lattice(Q):
     bb = np.zeros(Q, np.int32)
     assert lib().orc_lattice(Q, _ptr(e), _ptr(w), _ptr(bb)) == 0
     return e, w, bb
+
+
+def feq(Q, p, u, v=0.0, w=0.0):
+    out = np.zeros(Q)
+    assert lib().orc_feq(Q, p, u, v, w, _ptr(out)) == 0
+    return out
 
 
 class RefState:
