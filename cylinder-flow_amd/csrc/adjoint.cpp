@@ -33,6 +33,7 @@ static double weight(int m, int n0, int n1, double scale) {
 }
 
 FusedArgs ctx_fused_args(const lbm_ctx *c);
+StepBufs step_bufs(const lbm_ctx *c, const void *A, void *B);
 
 extern "C" int lbm_adjoint(lbm_ctx *c, int n0, int n1, double scale, int K, double *J, double *dJ_ds,
                            double *dJ_duin, double *ms_forward, double *ms_adjoint) {
@@ -98,8 +99,8 @@ extern "C" int lbm_adjoint(lbm_ctx *c, int n0, int n1, double scale, int K, doub
             const int n = (m0 + K <= n1) ? K : n1 - m0;
             ATRY(hipMemcpyAsync(store, (char *)ckpt + SZ * k, SZ, hipMemcpyDeviceToDevice, c->stream));
             for (int t = 0; t + 1 < n; t++)
-                ATRY(plain_step(Q, c->prec, (char *)store + SZ * t, (char *)store + SZ * (t + 1), c->flags, c->dsfrac,
-                                c->duin, c->block_slot, c->partials, c->macros, false, false, fa, c->stream));
+                ATRY(plain_step(Q, c->prec, c->V, step_bufs(c, (char *)store + SZ * t, (char *)store + SZ * (t + 1)),
+                                false, false, fa, c->stream));
             for (int t = n - 1; t >= 0; t--) {
                 const int m = m0 + t + 1;
                 AdjArgs aa;

@@ -61,6 +61,36 @@ __device__ __forceinline__ void zou_he(C *g, C U) {
     });
 }
 
+// Block-wide sum of three doubles in a fixed order (wave butterfly, then the waves in order);
+// the result is valid in thread 0.  `red` holds 3 doubles per wave of the block.
+template <int NW>
+__device__ __forceinline__ void block_sum3(double &a, double &b, double &c, double *red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+        c += __shfl_xor(c, off);
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[3 * wv] = a;
+        red[3 * wv + 1] = b;
+        red[3 * wv + 2] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        a = red[0];
+        b = red[1];
+        c = red[2];
+        for (int k = 1; k < nw && k < NW; k++) {
+            a += red[3 * k];
+            b += red[3 * k + 1];
+            c += red[3 * k + 2];
+        }
+    }
+}
+
 // Pull of the Q populations of one node from buffer A with the plain-mode boundary rules
 // (first match wins):
 //   x == 0 and e_ix > 0 (inlet, x not periodic)   -> Zou-He unknown (left 0, filled later)
@@ -69,6 +99,9 @@ __device__ __forceinline__ void zou_he(C *g, C U) {
 //   source is solid (mask)                         -> half-way bounce-back, momentum exchange
 //   otherwise                                      -> g_i = h_i(node - e_i) (periodic wrap)
 // FORCE accumulates the momentum exchange 2 h_opp(i) e_opp(i) of the mask links.
+// `node` must be a valid element of the plane even when the lane is not live; every load
+// below is issued unconditionally from a valid address (no data-dependent branches), so the
+// Q loads of a lane are in flight together.
 template <int Q, class S, bool FORCE>
 __device__ __forceinline__ void pull(const typename S::T *__restrict__ A, const uint8_t *__restrict__ flags,
                                      const Geo &g, const StoreParams &sp, int x, int y, int z, uint32_t node,
@@ -76,15 +109,17 @@ __device__ __forceinline__ void pull(const typename S::T *__restrict__ A, const 
     using L = Lat<Q>;
     using T = typename S::T;
     using C = typename S::C;
+    uint32_t src[Q];
+    bool zh[Q], wall[Q], solid[Q];
     static_for<0, Q>([&](auto I) {
         constexpr int i = I;
-        constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i), o = L::opp(i);
+        constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i);
         int sx = x - ex, sy = y - ey, sz = z - ez;
-        bool zh = false, bb = false, link = false;
+        bool z_ = false, w_ = false;
         if constexpr (ex > 0) {
             if (x == 0) {
                 if (g.xper) sx = g.nx - 1;
-                else { zh = true; sx = 0; }
+                else z_ = true;
             }
         }
         if constexpr (ex < 0) {
@@ -93,27 +128,41 @@ __device__ __forceinline__ void pull(const typename S::T *__restrict__ A, const 
         if constexpr (ey != 0) {
             if ((unsigned)sy >= (unsigned)g.ny) {
                 if (g.yper) sy = ey > 0 ? g.ny - 1 : 0;
-                else bb = true;
+                else w_ = true;
             }
         }
         if constexpr (ez != 0) {
             if ((unsigned)sz >= (unsigned)g.nz) {
                 if (g.zper) sz = ez > 0 ? g.nz - 1 : 0;
-                else bb = true;
+                else w_ = true;
             }
         }
-        const uint32_t src = (uint32_t)sx + (uint32_t)g.nxp * ((uint32_t)sy + (uint32_t)g.ny * (uint32_t)sz);
-        if (near && !zh && !bb) {
-            if (flags[src] == FL_SOLID) bb = link = true;
-        }
-        C v = 0;
-        if (live && !zh) {
-            const T *p = bb ? (A + (size_t)o * g.plane + node) : (A + (size_t)i * g.plane + src);
-            v = S::template decode<i, L>(*p, bb ? sp.off[o] : sp.off[i]);
-        }
+        zh[i] = z_;
+        wall[i] = w_ && !z_;
+        const bool plain = live && !z_ && !w_;
+        src[i] = plain ? (uint32_t)sx + (uint32_t)g.nxp * ((uint32_t)sy + (uint32_t)g.ny * (uint32_t)sz) : node;
+        solid[i] = false;
+    });
+    if (near) {   // only nodes next to the body look at their sources' flags
+        uint8_t fl[Q];
+        static_for<0, Q>([&](auto I) { fl[I] = flags[src[I]]; });
+        static_for<0, Q>([&](auto I) { solid[I] = !zh[I] && !wall[I] && fl[I] == FL_SOLID; });
+    }
+    T raw[Q];
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = I;
+        constexpr int o = L::opp(i);
+        const bool bb = wall[i] || solid[i];
+        raw[i] = bb ? A[(size_t)o * g.plane + node] : A[(size_t)i * g.plane + src[i]];
+    });
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = I;
+        constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i), o = L::opp(i);
+        const bool bb = wall[i] || solid[i];
+        const C v = (live && !zh[i]) ? S::template decode<i, L>(raw[i], bb ? sp.off[o] : sp.off[i]) : (C)0;
         gv[i] = v;
         if constexpr (FORCE) {
-            if (link) {
+            if (solid[i]) {
                 // momentum exchange of the bounced population h_opp(i): 2 h e_opp(i)
                 const double m = 2.0 * (double)v;
                 if constexpr (ex != 0) fx -= ex * m;
@@ -122,7 +171,6 @@ __device__ __forceinline__ void pull(const typename S::T *__restrict__ A, const 
             }
         }
     });
-
 }
 
 }  // namespace lbm

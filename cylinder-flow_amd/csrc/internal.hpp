@@ -8,7 +8,7 @@
 #include <thread>
 #include <vector>
 
-#include "../../include/lbm_capi.h"
+#include "lbm_capi.h"
 #include "store.hpp"
 
 namespace lbm {
@@ -40,9 +40,27 @@ Shape node_shape(const Geo &g);
 
 hipError_t plain_init(int Q, int prec, void *A, void *B, double *macros, const uint8_t *flags, const double *uin,
                       const double *xi, double perturb, uint64_t seed, const FusedArgs &a, hipStream_t s);
-hipError_t plain_step(int Q, int prec, const void *A, void *B, const uint8_t *flags, const void *sfrac,
-                      const double *uin, const int *block_slot, double *partials, double *macros,
-                      bool force, bool macros_on, const FusedArgs &a, hipStream_t s);
+// launch shape of the fused kernels: V nodes per lane, at most 256 lanes per block
+Shape lane_shape(const Geo &g, int V);
+
+// buffers of one fused step
+struct StepBufs {
+    const void *A;              // populations in (storage type)
+    void *B;                    // populations out
+    const uint8_t *flags;       // FL_* per node (padded layout)
+    const uint8_t *segslow;     // per (row, 64-lane wave): 1 = per-node path (FP32/16-bit kernels)
+    int nseg;                   // waves per row
+    const void *sfrac;          // solid fraction (FP64: double, else float) or null
+    const double *uin;          // inlet velocity per (y, z)
+    const int *block_slot;      // force partial slot per block (-1: none)
+    double *partials;
+    double *macros;             // p, u, v, w planes (written when macros_on)
+};
+hipError_t plain_step(int Q, int prec, int V, const StepBufs &b, bool force, bool macros_on, const FusedArgs &a,
+                      hipStream_t s);
+hipError_t fast_step_f32(int Q, int V, const StepBufs &b, bool force, bool mac, const FusedArgs &a, hipStream_t s);
+hipError_t fast_step_f16(int Q, int V, const StepBufs &b, bool force, bool mac, const FusedArgs &a, hipStream_t s);
+hipError_t fast_step_bf16(int Q, int V, const StepBufs &b, bool force, bool mac, const FusedArgs &a, hipStream_t s);
 hipError_t force_reduce(const double *partials, int nslots, double *hist, int rec, long step, hipStream_t s);
 // FP64 <-> storage conversion of one direction plane (physical nodes, x fastest)
 hipError_t plain_decode(int Q, int prec, const void *A, double *out, const uint8_t *flags, int dir,
@@ -114,6 +132,9 @@ struct lbm_ctx {
     int *block_slot = nullptr;
     double *partials = nullptr;
     int nslots = 0;
+    uint8_t *segslow = nullptr;     // per (row, wave): per-node path of the fused kernel
+    int nseg = 0;
+    int V = 1;                      // nodes per lane of the fused kernel
 
     // force history (device + host copy)
     double *hist = nullptr;

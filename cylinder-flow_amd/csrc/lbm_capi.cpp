@@ -6,6 +6,7 @@
 // lbm_create + lbm_initialize; its time loop body becomes lbm_step.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "internal.hpp"
@@ -88,7 +89,7 @@ extern "C" int lbm_config_default(lbm_config *c, int lattice) {
 extern "C" const char *lbm_last_error(const lbm_ctx *ctx) { return ctx ? ctx->err.c_str() : t_create_err.c_str(); }
 
 static void free_all(lbm_ctx *c) {
-    void *ptrs[] = {c->A, c->B, c->flags, c->dsfrac, c->duin, c->macros, c->block_slot, c->partials, c->hist,
+    void *ptrs[] = {c->A, c->B, c->flags, c->dsfrac, c->duin, c->macros, c->block_slot, c->partials, c->hist, c->segslow,
                     c->rf, c->rft, c->rmac, c->rprof, c->rusr, c->rfpre};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -97,6 +98,7 @@ static void free_all(lbm_ctx *c) {
     c->dsfrac = nullptr;
     c->duin = c->macros = c->partials = c->hist = nullptr;
     c->block_slot = nullptr;
+    c->segslow = nullptr;
     c->rf = c->rft = c->rmac = c->rprof = c->rfpre = nullptr;
     c->rusr = nullptr;
 }
@@ -145,6 +147,15 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_ctx **out) {
     c->geo.yper = plain && cfg->periodic[1];
     c->geo.zper = plain && Q == 27 && cfg->periodic[2];
     c->N = (size_t)cfg->nx * cfg->ny * nz;
+    // nodes per lane of the fused kernel: one 4-byte access per direction for two FP16/BF16
+    // nodes, 8 bytes for two FP32 nodes; FP64 keeps one node per lane.  LBM_NODES_PER_LANE
+    // overrides it among the instantiated widths (FP32: 1, 2; FP16/BF16: 2, 4).
+    c->V = c->prec == LBM_FP64 ? 1 : 2;
+    if (const char *env = std::getenv("LBM_NODES_PER_LANE")) {
+        const int v = std::atoi(env);
+        const bool ok = c->prec == LBM_FP32 ? (v == 1 || v == 2) : c->prec != LBM_FP64 && (v == 2 || v == 4);
+        if (ok) c->V = v;
+    }
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         set_err(nullptr, LBM_ERR_HIP, std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
@@ -353,22 +364,44 @@ static int init_plain(lbm_ctx *c) {
                 c->near_nodes += near;
             }
     // force partial slots: blocks holding a node next to the mask or with a solid fraction
-    const Shape sh = node_shape(g);
+    const int V = c->V;
+    const Shape sh = lane_shape(g, V);
     const size_t nblocks = (size_t)sh.gx * g.ny * g.nz;
     std::vector<int> slot(nblocks, -1);
     c->nslots = 0;
     const bool sf = !c->sfrac.empty();
+    auto special = [&](int x, int y, int z) {   // a node the fast path may not handle
+        if (x >= g.nx) return true;
+        const size_t node = (size_t)x + (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
+        if (flags[node] != FL_FLUID) return true;
+        if (sf && c->sfrac[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] > 0.0) return true;
+        return g.xper && (x == 0 || x == g.nx - 1);
+    };
     for (int z = 0; z < g.nz; z++)
         for (int y = 0; y < g.ny; y++)
             for (int bxi = 0; bxi < sh.gx; bxi++) {
                 bool any = false;
-                for (int x = bxi * sh.bx; x < std::min(g.nx, (bxi + 1) * sh.bx) && !any; x++) {
+                for (int x = bxi * sh.bx * V; x < std::min(g.nx, (bxi + 1) * sh.bx * V) && !any; x++) {
                     const size_t node = (size_t)x + (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
                     any = flags[node] == FL_NEAR ||
                           (sf && flags[node] != FL_SOLID && c->sfrac[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] > 0.0);
                 }
                 if (any) slot[(size_t)bxi + (size_t)sh.gx * ((size_t)y + (size_t)g.ny * z)] = c->nslots++;
             }
+    // per (row, 64-lane wave): 1 sends the wave through the per-node path (fused FP32/16-bit kernel)
+    c->nseg = (sh.gx * sh.bx + 63) / 64;
+    const char *force_slow = std::getenv("LBM_FORCE_SLOW_PATH");
+    const bool all_slow = force_slow && force_slow[0] == '1';
+    std::vector<uint8_t> segslow((size_t)c->nseg * g.ny * g.nz, 0);
+    for (int z = 0; z < g.nz; z++)
+        for (int y = 0; y < g.ny; y++)
+            for (int s = 0; s < c->nseg; s++) {
+                bool slow = all_slow;
+                for (int x = s * 64 * V; x < (s + 1) * 64 * V && !slow; x++) slow = special(x, y, z);
+                segslow[(size_t)(y + (size_t)g.ny * z) * c->nseg + s] = slow;
+            }
+    HIP_TRY(c, hipMalloc(&c->segslow, segslow.size()));
+    HIP_TRY(c, hipMemcpyAsync(c->segslow, segslow.data(), segslow.size(), hipMemcpyHostToDevice, c->stream));
     const size_t esz = c->prec == LBM_FP64 ? 8 : c->prec == LBM_FP32 ? 4 : 2;
     HIP_TRY(c, hipMalloc(&c->A, esz * Q * g.plane));
     HIP_TRY(c, hipMalloc(&c->B, esz * Q * g.plane));
@@ -473,6 +506,21 @@ extern "C" int lbm_initialize(lbm_ctx *c) {
     return LBM_OK;
 }
 
+StepBufs step_bufs(const lbm_ctx *c, const void *A, void *B) {
+    StepBufs b;
+    b.A = A;
+    b.B = B;
+    b.flags = c->flags;
+    b.segslow = c->segslow;
+    b.nseg = c->nseg;
+    b.sfrac = c->dsfrac;
+    b.uin = c->duin;
+    b.block_slot = c->block_slot;
+    b.partials = c->partials;
+    b.macros = c->macros;
+    return b;
+}
+
 static int step_n(lbm_ctx *c, int n) {
     const int fi = c->cfg.force_interval;
     const int nforce = fi > 0 ? (int)((c->step + n) / fi - c->step / fi) : 0;
@@ -495,8 +543,8 @@ static int step_n(lbm_ctx *c, int n) {
     for (int k = 0; k < n; k++) {
         c->step++;
         const bool force = fi > 0 && c->step % fi == 0;
-        HIP_TRY(c, plain_step(c->Q, c->prec, c->A, c->B, c->flags, c->dsfrac, c->duin, c->block_slot, c->partials,
-                              c->macros, force, k == n - 1, a, c->stream));
+        const StepBufs b = step_bufs(c, c->A, c->B);
+        HIP_TRY(c, plain_step(c->Q, c->prec, c->V, b, force, k == n - 1, a, c->stream));
         if (force) HIP_TRY(c, force_reduce(c->partials, c->nslots, c->hist, c->nrec++, c->step, c->stream));
         std::swap(c->A, c->B);
     }

@@ -31,32 +31,6 @@ Shape node_shape(const Geo &g) {
     return s;
 }
 
-template <int NW>
-__device__ __forceinline__ void block_sum3(double &a, double &b, double &c, double *red) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        a += __shfl_xor(a, off);
-        b += __shfl_xor(b, off);
-        c += __shfl_xor(c, off);
-    }
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) {
-        red[3 * wv] = a;
-        red[3 * wv + 1] = b;
-        red[3 * wv + 2] = c;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int nw = (blockDim.x + 63) >> 6;
-        a = red[0]; b = red[1]; c = red[2];
-        for (int k = 1; k < nw; k++) {
-            a += red[3 * k];
-            b += red[3 * k + 1];
-            c += red[3 * k + 2];
-        }
-    }
-}
-
 template <int Q, class S>
 __global__ void __launch_bounds__(512) k_init(typename S::T *__restrict__ A, typename S::T *__restrict__ B,
                                               double *__restrict__ macros, const uint8_t *__restrict__ flags, const double *__restrict__ uin,
@@ -102,7 +76,8 @@ __global__ void __launch_bounds__(512)
     const int y = blockIdx.y, z = blockIdx.z;
     const bool active = x < g.nx;
     // element offset inside one direction plane (a plane is < 2^32 elements)
-    const uint32_t node = (uint32_t)x + (uint32_t)g.nxp * ((uint32_t)y + (uint32_t)g.ny * (uint32_t)z);
+    // lanes past the row (x >= nx) use the row's first element as a valid, unused address
+    const uint32_t node = (uint32_t)(active ? x : 0) + (uint32_t)g.nxp * ((uint32_t)y + (uint32_t)g.ny * (uint32_t)z);
     const uint8_t fl = active ? flags[node] : (uint8_t)FL_SOLID;
     const bool live = fl != FL_SOLID;
     const bool near = fl == FL_NEAR;
@@ -287,40 +262,47 @@ hipError_t plain_init(int Q, int prec, void *A, void *B, double *macros, const u
                   : init_q<27>(prec, A, B, macros, flags, uin, xi, perturb, seed, a, s);
 }
 
-template <int Q, class S, bool FORCE, bool MACROS>
-static hipError_t step_qs(const void *A, void *B, const uint8_t *flags, const void *sfrac, const double *uin,
-                          const int *block_slot, double *partials, double *macros, const FusedArgs &a,
-                          hipStream_t s) {
-    using T = typename S::T;
-    using C = typename S::C;
-    const Shape sh = node_shape(a.g);
-    if (sfrac)
+Shape lane_shape(const Geo &g, int V) {
+    Shape s;
+    const int lanes = g.nxp / V;
+    s.bx = lanes <= 256 ? lanes : 256;
+    s.gx = (lanes + s.bx - 1) / s.bx;
+    return s;
+}
+
+// FP64 storage: one node per lane, the reference's rounding order
+template <int Q, bool FORCE, bool MACROS>
+static hipError_t step_f64(const StepBufs &b, const FusedArgs &a, hipStream_t s) {
+    using S = SF64;
+    const Shape sh = lane_shape(a.g, 1);
+    if (b.sfrac)
         hipLaunchKernelGGL((k_fused<Q, S, FORCE, MACROS, true>), grid_of(a.g, sh), dim3(sh.bx), 0, s,
-                           (const T *)A, (T *)B, flags, (const C *)sfrac, uin, block_slot, partials, macros, a);
+                           (const double *)b.A, (double *)b.B, b.flags, (const double *)b.sfrac, b.uin, b.block_slot,
+                           b.partials, b.macros, a);
     else
         hipLaunchKernelGGL((k_fused<Q, S, FORCE, MACROS, false>), grid_of(a.g, sh), dim3(sh.bx), 0, s,
-                           (const T *)A, (T *)B, flags, (const C *)nullptr, uin, block_slot, partials, macros, a);
+                           (const double *)b.A, (double *)b.B, b.flags, (const double *)nullptr, b.uin, b.block_slot,
+                           b.partials, b.macros, a);
     return hipGetLastError();
 }
 
 template <int Q>
-static hipError_t step_q(int prec, const void *A, void *B, const uint8_t *flags, const void *sfrac,
-                         const double *uin, const int *block_slot, double *partials, double *macros, bool force,
-                         bool mac, const FusedArgs &a, hipStream_t s) {
-    return dispatch_store(prec, [&](auto st) {
-        using S = decltype(st);
-        if (force && mac) return step_qs<Q, S, true, true>(A, B, flags, sfrac, uin, block_slot, partials, macros, a, s);
-        if (force) return step_qs<Q, S, true, false>(A, B, flags, sfrac, uin, block_slot, partials, macros, a, s);
-        if (mac) return step_qs<Q, S, false, true>(A, B, flags, sfrac, uin, block_slot, partials, macros, a, s);
-        return step_qs<Q, S, false, false>(A, B, flags, sfrac, uin, block_slot, partials, macros, a, s);
-    });
+static hipError_t step_f64_q(const StepBufs &b, bool force, bool mac, const FusedArgs &a, hipStream_t s) {
+    if (force && mac) return step_f64<Q, true, true>(b, a, s);
+    if (force) return step_f64<Q, true, false>(b, a, s);
+    if (mac) return step_f64<Q, false, true>(b, a, s);
+    return step_f64<Q, false, false>(b, a, s);
 }
 
-hipError_t plain_step(int Q, int prec, const void *A, void *B, const uint8_t *flags, const void *sfrac,
-                      const double *uin, const int *block_slot, double *partials, double *macros, bool force,
-                      bool macros_on, const FusedArgs &a, hipStream_t s) {
-    return Q == 9 ? step_q<9>(prec, A, B, flags, sfrac, uin, block_slot, partials, macros, force, macros_on, a, s)
-                  : step_q<27>(prec, A, B, flags, sfrac, uin, block_slot, partials, macros, force, macros_on, a, s);
+hipError_t plain_step(int Q, int prec, int V, const StepBufs &b, bool force, bool macros_on, const FusedArgs &a,
+                      hipStream_t s) {
+    switch (prec) {
+    case LBM_FP64: return Q == 9 ? step_f64_q<9>(b, force, macros_on, a, s) : step_f64_q<27>(b, force, macros_on, a, s);
+    case LBM_FP32: return fast_step_f32(Q, V, b, force, macros_on, a, s);
+    case LBM_FP16S: return fast_step_f16(Q, V, b, force, macros_on, a, s);
+    case LBM_BF16S: return fast_step_bf16(Q, V, b, force, macros_on, a, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t force_reduce(const double *partials, int nslots, double *hist, int rec, long step, hipStream_t s) {

@@ -27,6 +27,8 @@ struct SF64 {
     __device__ __forceinline__ static C decode(T h, float) { return h; }
     __device__ __forceinline__ static C decode(T h, float, float) { return h; }
     template <int I, class L>
+    __device__ __forceinline__ static T enc(C v, const StoreParams &) { return v; }
+    template <int I, class L>
     __device__ __forceinline__ static void store(T *p, C v, const StoreParams &) { __builtin_nontemporal_store(v, p); }
     __device__ __forceinline__ static T encode(double v, float, float) { return v; }
 };
@@ -38,6 +40,8 @@ struct SF32 {
     template <int I, class L>
     __device__ __forceinline__ static C decode(T h, float) { return h; }
     __device__ __forceinline__ static C decode(T h, float, float) { return h; }
+    template <int I, class L>
+    __device__ __forceinline__ static T enc(C v, const StoreParams &) { return v; }
     template <int I, class L>
     __device__ __forceinline__ static void store(T *p, C v, const StoreParams &) { __builtin_nontemporal_store(v, p); }
     __device__ __forceinline__ static T encode(double v, float, float) { return (float)v; }
@@ -53,13 +57,23 @@ struct SF16 {
         return __builtin_fmaf((float)h, (float)L::w(I), off);
     }
     __device__ __forceinline__ static C decode(T h, float w, float off) { return __builtin_fmaf((float)h, w, off); }
+    // The empty asm keeps the FP32 result in a register, so the compiler cannot fuse the FMA
+    // and the f16 conversion into one mixed-precision instruction: every value is rounded to
+    // FP32 and then to FP16 exactly as written, whichever code path encodes it.
+    __device__ __forceinline__ static T round16(float t) {
+        asm volatile("" : "+v"(t));
+        return (T)t;
+    }
+    template <int I, class L>
+    __device__ __forceinline__ static T enc(C v, const StoreParams &sp) {
+        return round16(__builtin_fmaf(v, (float)(1.0 / L::w(I)), -sp.offw[I]));
+    }
     template <int I, class L>
     __device__ __forceinline__ static void store(T *p, C v, const StoreParams &sp) {
-        const T h = (T)__builtin_fmaf(v, (float)(1.0 / L::w(I)), -sp.offw[I]);
-        __builtin_nontemporal_store(h, p);
+        __builtin_nontemporal_store(enc<I, L>(v, sp), p);
     }
     __device__ __forceinline__ static T encode(double v, float winv, float offw) {
-        return (T)__builtin_fmaf((float)v, winv, -offw);
+        return round16(__builtin_fmaf((float)v, winv, -offw));
     }
 };
 
@@ -80,8 +94,12 @@ struct SBF16 {
     }
     __device__ __forceinline__ static C decode(T h, float w, float off) { return __builtin_fmaf(widen(h), w, off); }
     template <int I, class L>
+    __device__ __forceinline__ static T enc(C v, const StoreParams &sp) {
+        return narrow(__builtin_fmaf(v, (float)(1.0 / L::w(I)), -sp.offw[I]));
+    }
+    template <int I, class L>
     __device__ __forceinline__ static void store(T *p, C v, const StoreParams &sp) {
-        __builtin_nontemporal_store(narrow(__builtin_fmaf(v, (float)(1.0 / L::w(I)), -sp.offw[I])), p);
+        __builtin_nontemporal_store(enc<I, L>(v, sp), p);
     }
     __device__ __forceinline__ static T encode(double v, float winv, float offw) {
         return narrow(__builtin_fmaf((float)v, winv, -offw));

@@ -113,3 +113,21 @@ def test_system_2d_hard_coded_parameters():
     vals = dict(re.findall(r"^(\w+)\s*=(\S+)$", out, flags=re.M))
     assert vals["Time"] == "1000" and vals["inter"] == "100"
     assert float(vals["tau"]) == pytest.approx(0.1 * 12 / 100 * 3)
+
+
+def test_legendre_basis_matches_oracle_inlet(pkg, orc):
+    from cylinder_flow_amd.autograd import legendre_basis
+    coef = pkg.configs.legendre_coefficients()
+    assert abs(coef).sum() <= 0.05 + 1e-15 and coef[0] > 0
+    ny, nz = 12, 10
+    cfg = orc.make_cfg(27, 8, ny, nz, tau0=0.1, u0=0.05, inlet=orc.INLET_LEGENDRE, coef=tuple(coef))
+    uin = orc.PlainState(cfg).inlet()
+    phi = legendre_basis(ny, nz).numpy()
+    assert np.allclose(np.einsum("k,kzy->zy", coef, phi), uin, atol=1e-15)
+
+
+def test_strouhal_of_a_sine(pkg):
+    t = np.arange(4000)
+    cl = 0.3 * np.sin(2 * np.pi * t / 250.0 + 0.4)
+    # period 250 steps sampled every step: St = D / (T u) with D = 20, u = 0.05
+    assert abs(pkg.configs.strouhal(cl, 1.0, 20.0, 0.05) - 20.0 / (250.0 * 0.05)) < 1e-3
