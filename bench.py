@@ -114,15 +114,29 @@ def adjoint_ratio(pkg, steps=500):
 
 
 def copy_calibration(n_bytes=1 << 30, reps=3):
-    """A plain device-to-device copy of known size (read n_bytes + write n_bytes): its
-    FETCH_SIZE / WRITE_SIZE counters give the scale factor from counter units to bytes."""
-    import torch
-    a = torch.empty(n_bytes // 4, dtype=torch.float32, device="cuda")
-    a.fill_(1.0)
-    b = torch.empty_like(a)
-    for _ in range(reps):
-        b.copy_(a)
-    torch.cuda.synchronize()
+    """A plain device-to-device copy of known size (read n_bytes + write n_bytes) through the
+    HIP runtime (its copy kernel): that dispatch's FETCH_SIZE / WRITE_SIZE counters give the
+    scale factor from counter units to bytes."""
+    import ctypes as C
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(n_bytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, C.c_size_t(n_bytes)) != 0:
+            raise RuntimeError("hipMemset failed")
+        for _ in range(reps):
+            if hip.hipMemcpy(b, a, C.c_size_t(n_bytes), 3) != 0:   # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed")
+        if hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipDeviceSynchronize failed")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
     return {"bytes_read": n_bytes, "bytes_written": n_bytes, "reps": reps}
 
 
@@ -163,7 +177,7 @@ def main():
                    "force_interval": 10, "body": "z-cylinder D=40", "Re": 300},
         "roofline": {"bound": "hbm", "achieved": round(ach16, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach16 / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles("fp16s"),
-                     "kernel": "k_fused<27,SF16,false,false,false>", "kernel_ms": round(t16 * 1e3, 5),
+                     "kernel": f"k_fused_v<27,SF16,V={info16.nodes_per_lane}>", "kernel_ms": round(t16 * 1e3, 5),
                      "bytes_per_update": info16.bytes_per_update},
         "forces_finite": bool(forces.size and (abs(forces[:, 1:]) < 1e30).all()),
     }
@@ -177,7 +191,7 @@ def main():
                        "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS,
                                     "unit": "GB/s", "frac": round(ach32 / HBM_PEAK_GBS, 4),
                                     "traffic": traffic_from_profiles("fp32"),
-                                    "kernel": "k_fused<27,SF32,false,false,false>",
+                                    "kernel": f"k_fused_v<27,SF32,V={info32.nodes_per_lane}>",
                                     "kernel_ms": round(t32 * 1e3, 5), "bytes_per_update": info32.bytes_per_update}}
     if not a.no_adjoint:
         log("adjoint leg")

@@ -1,7 +1,7 @@
 // adjoint_kernels.hip -- discrete adjoint of the plain-mode step (no reference counterpart).
 //
 // Forward step m (1-based) maps H_{m-1} (post-collision populations) to G_m (pulled
-// populations after the boundary rules and Zou-He) and H_m = (1-s) C(G_m) + s G_m[opp]
+// populations after the boundary rules and the regularised Zou-He inlet) and H_m = (1-s) C(G_m) + s G_m[opp]
 // (BGK collision C, partial bounce-back with solid fraction s).  Its drag is
 //   Fx_m = sum over mask links of 2 h_opp(i) e_opp(i),x   (from H_{m-1})
 //        + sum over nodes of 2 s u_x(G_m)                 (partial bounce-back).
@@ -10,9 +10,10 @@
 //   1. lambda(a) = dJ/dH_m(a): gather of mu_{m+1} over every pulled entry that read a
 //      population of node a (transpose of the pull, evaluated as a pull along +e),
 //      plus the link-force term of step m+1;
-//   2. recompute G_m(a) from H_{m-1} with the forward's own pull and Zou-He;
+//   2. recompute G_m(a) from H_{m-1} with the forward's own pull and velocity inlet;
 //   3. mu_m(a) = (collision + partial bounce-back)^T lambda + alpha_m dFx/dG, then the
-//      Zou-He transpose at inlet nodes; accumulate dJ/ds(a) and dJ/du_in(row).
+//      transposes of the regularisation and of Zou-He at inlet nodes; accumulate dJ/ds(a)
+//      and dJ/du_in(row).
 // Traffic per node: Q reads of H, Q reads of mu_{m+1}, Q writes of mu_m.
 #include "fused_common.hpp"
 
@@ -101,7 +102,7 @@ __global__ void __launch_bounds__(256)
     double fx = 0.0, fy = 0.0, fz = 0.0;
     pull<Q, S, false>(H, flags, g, a.f.sp, x, y, z, node, true, fl == FL_NEAR, gv, fx, fy, fz);
     const bool inlet = !g.xper && x == 0;
-    if (inlet) zou_he<Q, C>(gv, (C)uin[y + g.ny * z]);
+    if (inlet) velocity_inlet<Q, C>(gv, (C)uin[y + g.ny * z]);
 
     // ---- 3. local transpose ----
     C p = 0, u[3] = {0, 0, 0};
@@ -149,8 +150,42 @@ __global__ void __launch_bounds__(256)
                al * (C)2 * sf * (C)L::ex(j);
     });
     if (inlet) {
-        // g_i = g_opp(i) + 6 w_i U - 18 w_i (e_iy N_y + e_iz N_z), N_t = sum_{e_kx=0} g_k e_kt
+        // regularisation g_i = w_i (S + 3 e_ix U + 4.5 Q_i:P) is g -> A g + 3 w e_x U with
+        // A_ij = w_i (1 + 4.5 Q_i:Q_j), so (A^T m)_j = M0 + 4.5 Q_j:T, M0 = sum w_i m_i,
+        // T = sum w_i m_i Q_i (same structure as the forward with S -> M0, P -> T)
         double du = 0.0;
+        {
+            C M0 = 0, Rxx = 0, Ryy = 0, Rzz = 0, Rxy = 0, Rxz = 0, Ryz = 0;
+            static_for<0, Q>([&](auto I) {
+                constexpr int i = I;
+                constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i);
+                const C wm = (C)L::w(i) * m[i];
+                M0 += wm;
+                if constexpr (ex != 0) Rxx += wm;
+                if constexpr (ey != 0) Ryy += wm;
+                if constexpr (ez != 0) Rzz += wm;
+                if constexpr (ex * ey != 0) Rxy += (C)(ex * ey) * wm;
+                if constexpr (ex * ez != 0) Rxz += (C)(ex * ez) * wm;
+                if constexpr (ey * ez != 0) Ryz += (C)(ey * ez) * wm;
+                if constexpr (ex != 0) du += (double)wm * (3.0 * ex);
+            });
+            const C M3 = M0 * (C)(1.0 / 3.0);
+            const C Txx = Rxx - M3, Tyy = Ryy - M3, Tzz = Rzz - M3;
+            const C d = (L::D == 3 ? Txx + Tyy + Tzz : Txx + Tyy) * (C)(1.0 / 3.0);
+            static_for<0, Q>([&](auto J) {
+                constexpr int j = J;
+                constexpr int ex = L::ex(j), ey = L::ey(j), ez = L::ez(j);
+                C q = -d, o = 0;
+                if constexpr (ex != 0) q += Txx;
+                if constexpr (ey != 0) q += Tyy;
+                if constexpr (ez != 0) q += Tzz;
+                if constexpr (ex * ey != 0) o += (C)(ex * ey) * Rxy;
+                if constexpr (ex * ez != 0) o += (C)(ex * ez) * Rxz;
+                if constexpr (ey * ez != 0) o += (C)(ey * ez) * Ryz;
+                m[j] = M0 + ((C)4.5 * q + (C)9 * o);
+            });
+        }
+        // Zou-He g_i = g_opp(i) + 6 w_i U - 18 w_i (e_iy N_y + e_iz N_z), N_t = sum_{e_kx=0} g_k e_kt
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
             if constexpr (L::ex(i) > 0) {
@@ -180,7 +215,6 @@ __global__ void __launch_bounds__(256)
                const double *__restrict__ xi, double perturb, uint64_t seed, double alpha1, double *__restrict__ dJdu,
                const Geo g) {
     using L = Lat<Q>;
-    using C = typename S::C;
     const int y = blockIdx.y, z = blockIdx.z;
     double acc = 0.0;
     for (int x = threadIdx.x; x < g.nx; x += blockDim.x) {

@@ -32,6 +32,15 @@ struct FusedArgs {
 // node flags
 enum : uint8_t { FL_FLUID = 0, FL_SOLID = 1, FL_NEAR = 2 /* fluid with a solid neighbour */ };
 
+// wave classes of the FP32/16-bit fused kernel, one per (row, 64-lane wave):
+//   WAVE_FAST  every node fluid with fluid sources, no padding, x not periodic at this wave
+//   WAVE_NEAR  as WAVE_FAST but with solid nodes or mask links: packed loads plus a per-node
+//              link word (bit i: direction i pulls from a solid node; LINK_SOLID: the node
+//              itself is solid and is neither updated nor stored)
+//   WAVE_NODE  per-node path (padding columns, periodic x edge, solid fraction)
+enum : uint8_t { WAVE_FAST = 0, WAVE_NEAR = 1, WAVE_NODE = 2 };
+constexpr uint32_t LINK_SOLID = 1u << 31;
+
 // launch shape of every per-node kernel: blocks of `bx` threads along x, `gx` per row
 struct Shape {
     int bx, gx;
@@ -48,8 +57,9 @@ struct StepBufs {
     const void *A;              // populations in (storage type)
     void *B;                    // populations out
     const uint8_t *flags;       // FL_* per node (padded layout)
-    const uint8_t *segslow;     // per (row, 64-lane wave): 1 = per-node path (FP32/16-bit kernels)
+    const uint8_t *segcls;      // per (row, 64-lane wave): a WAVE_* class (FP32/16-bit kernels)
     int nseg;                   // waves per row
+    const uint32_t *links;      // per node: LINK_* bits (null when no wave is WAVE_NEAR)
     const void *sfrac;          // solid fraction (FP64: double, else float) or null
     const double *uin;          // inlet velocity per (y, z)
     const int *block_slot;      // force partial slot per block (-1: none)
@@ -132,8 +142,9 @@ struct lbm_ctx {
     int *block_slot = nullptr;
     double *partials = nullptr;
     int nslots = 0;
-    uint8_t *segslow = nullptr;     // per (row, wave): per-node path of the fused kernel
+    uint8_t *segcls = nullptr;      // per (row, wave): WAVE_* class of the fused kernel
     int nseg = 0;
+    uint32_t *links = nullptr;      // per node: LINK_* bits, when some wave is WAVE_NEAR
     int V = 1;                      // nodes per lane of the fused kernel
 
     // force history (device + host copy)

@@ -89,8 +89,8 @@ extern "C" int lbm_config_default(lbm_config *c, int lattice) {
 extern "C" const char *lbm_last_error(const lbm_ctx *ctx) { return ctx ? ctx->err.c_str() : t_create_err.c_str(); }
 
 static void free_all(lbm_ctx *c) {
-    void *ptrs[] = {c->A, c->B, c->flags, c->dsfrac, c->duin, c->macros, c->block_slot, c->partials, c->hist, c->segslow,
-                    c->rf, c->rft, c->rmac, c->rprof, c->rusr, c->rfpre};
+    void *ptrs[] = {c->A, c->B, c->flags, c->dsfrac, c->duin, c->macros, c->block_slot, c->partials, c->hist, c->segcls,
+                    c->links, c->rf, c->rft, c->rmac, c->rprof, c->rusr, c->rfpre};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c->A = c->B = nullptr;
@@ -98,8 +98,9 @@ static void free_all(lbm_ctx *c) {
     c->dsfrac = nullptr;
     c->duin = c->macros = c->partials = c->hist = nullptr;
     c->block_slot = nullptr;
-    c->segslow = nullptr;
-    c->rf = c->rft = c->rmac = c->rprof = c->rfpre = nullptr;
+    c->segcls = nullptr;
+    c->links = nullptr;
+    c->rf =c->rft = c->rmac = c->rprof = c->rfpre = nullptr;
     c->rusr = nullptr;
 }
 
@@ -148,9 +149,9 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_ctx **out) {
     c->geo.zper = plain && Q == 27 && cfg->periodic[2];
     c->N = (size_t)cfg->nx * cfg->ny * nz;
     // nodes per lane of the fused kernel: one 4-byte access per direction for two FP16/BF16
-    // nodes, 8 bytes for two FP32 nodes; FP64 keeps one node per lane.  LBM_NODES_PER_LANE
-    // overrides it among the instantiated widths (FP32: 1, 2; FP16/BF16: 2, 4).
-    c->V = c->prec == LBM_FP64 ? 1 : 2;
+    // nodes; FP32 and FP64 keep one node per lane (measured fastest on the 512x256x256 case).
+    // LBM_NODES_PER_LANE overrides it among the instantiated widths (FP32: 1, 2; FP16/BF16: 2, 4).
+    c->V = (c->prec == LBM_FP16S || c->prec == LBM_BF16S) ? 2 : 1;
     if (const char *env = std::getenv("LBM_NODES_PER_LANE")) {
         const int v = std::atoi(env);
         const bool ok = c->prec == LBM_FP32 ? (v == 1 || v == 2) : c->prec != LBM_FP64 && (v == 2 || v == 4);
@@ -370,12 +371,13 @@ static int init_plain(lbm_ctx *c) {
     std::vector<int> slot(nblocks, -1);
     c->nslots = 0;
     const bool sf = !c->sfrac.empty();
-    auto special = [&](int x, int y, int z) {   // a node the fast path may not handle
-        if (x >= g.nx) return true;
+    auto node_class = [&](int x, int y, int z) -> int {   // the wave class a node needs
+        if (x >= g.nx) return WAVE_NODE;
+        if (g.xper && (x == 0 || x == g.nx - 1)) return WAVE_NODE;
         const size_t node = (size_t)x + (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
-        if (flags[node] != FL_FLUID) return true;
-        if (sf && c->sfrac[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] > 0.0) return true;
-        return g.xper && (x == 0 || x == g.nx - 1);
+        if (sf && flags[node] != FL_SOLID && c->sfrac[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] > 0.0)
+            return WAVE_NODE;
+        return flags[node] == FL_FLUID ? WAVE_FAST : WAVE_NEAR;
     };
     for (int z = 0; z < g.nz; z++)
         for (int y = 0; y < g.ny; y++)
@@ -388,20 +390,52 @@ static int init_plain(lbm_ctx *c) {
                 }
                 if (any) slot[(size_t)bxi + (size_t)sh.gx * ((size_t)y + (size_t)g.ny * z)] = c->nslots++;
             }
-    // per (row, 64-lane wave): 1 sends the wave through the per-node path (fused FP32/16-bit kernel)
+    // per (row, 64-lane wave): the most general class any of its nodes needs (fused FP32/16-bit
+    // kernel); LBM_FORCE_SLOW_PATH=1 sends every wave through the per-node path
     c->nseg = (sh.gx * sh.bx + 63) / 64;
     const char *force_slow = std::getenv("LBM_FORCE_SLOW_PATH");
     const bool all_slow = force_slow && force_slow[0] == '1';
-    std::vector<uint8_t> segslow((size_t)c->nseg * g.ny * g.nz, 0);
+    std::vector<uint8_t> segcls((size_t)c->nseg * g.ny * g.nz, WAVE_FAST);
+    bool any_near = false;
     for (int z = 0; z < g.nz; z++)
         for (int y = 0; y < g.ny; y++)
             for (int s = 0; s < c->nseg; s++) {
-                bool slow = all_slow;
-                for (int x = s * 64 * V; x < (s + 1) * 64 * V && !slow; x++) slow = special(x, y, z);
-                segslow[(size_t)(y + (size_t)g.ny * z) * c->nseg + s] = slow;
+                int cls = all_slow ? WAVE_NODE : WAVE_FAST;
+                for (int x = s * 64 * V; x < (s + 1) * 64 * V && cls != WAVE_NODE; x++)
+                    cls = std::max(cls, node_class(x, y, z));
+                segcls[(size_t)(y + (size_t)g.ny * z) * c->nseg + s] = (uint8_t)cls;
+                any_near |= cls == WAVE_NEAR;
             }
-    HIP_TRY(c, hipMalloc(&c->segslow, segslow.size()));
-    HIP_TRY(c, hipMemcpyAsync(c->segslow, segslow.data(), segslow.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMalloc(&c->segcls, segcls.size()));
+    HIP_TRY(c, hipMemcpyAsync(c->segcls, segcls.data(), segcls.size(), hipMemcpyHostToDevice, c->stream));
+    if (any_near) {
+        // link words: the same source classification as the per-node pull (Zou-He unknowns and
+        // wall links first, outlet clamp, periodic wrap), bit i set when the source is solid
+        std::vector<uint32_t> links(g.plane, 0);
+        for (int z = 0; z < g.nz; z++)
+            for (int y = 0; y < g.ny; y++)
+                for (int x = 0; x < g.nx; x++) {
+                    const size_t node = (size_t)x + (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
+                    if (flags[node] == FL_SOLID) {
+                        links[node] = LINK_SOLID;
+                        continue;
+                    }
+                    if (flags[node] != FL_NEAR) continue;
+                    uint32_t bits = 0;
+                    for (int i = 1; i < Q; i++) {
+                        const int ex = e[3 * i], ey = e[3 * i + 1], ez = e[3 * i + 2];
+                        if (ex > 0 && x == 0 && !g.xper) continue;
+                        int sx = x - ex, sy = y - ey, sz = z - ez;
+                        if ((sy < 0 || sy >= g.ny) && !g.yper) continue;
+                        if ((sz < 0 || sz >= g.nz) && !g.zper) continue;
+                        if (sx == g.nx && !g.xper) sx = g.nx - 1;
+                        if (solid_at(sx, sy, sz)) bits |= 1u << i;
+                    }
+                    links[node] = bits;
+                }
+        HIP_TRY(c, hipMalloc(&c->links, sizeof(uint32_t) * g.plane));
+        HIP_TRY(c, hipMemcpy(c->links, links.data(), sizeof(uint32_t) * g.plane, hipMemcpyHostToDevice));
+    }
     const size_t esz = c->prec == LBM_FP64 ? 8 : c->prec == LBM_FP32 ? 4 : 2;
     HIP_TRY(c, hipMalloc(&c->A, esz * Q * g.plane));
     HIP_TRY(c, hipMalloc(&c->B, esz * Q * g.plane));
@@ -511,8 +545,9 @@ StepBufs step_bufs(const lbm_ctx *c, const void *A, void *B) {
     b.A = A;
     b.B = B;
     b.flags = c->flags;
-    b.segslow = c->segslow;
+    b.segcls = c->segcls;
     b.nseg = c->nseg;
+    b.links = c->links;
     b.sfrac = c->dsfrac;
     b.uin = c->duin;
     b.block_slot = c->block_slot;
@@ -713,5 +748,6 @@ extern "C" int lbm_info_get(lbm_ctx *c, lbm_info *info) {
     info->step = c->step;
     info->solid_nodes = c->solid_nodes;
     info->boundary_nodes = c->near_nodes;
+    info->nodes_per_lane = c->cfg.bc_mode == LBM_BC_PLAIN ? c->V : 1;
     return LBM_OK;
 }

@@ -69,7 +69,6 @@ __global__ void __launch_bounds__(512)
             const int *__restrict__ block_slot, double *__restrict__ partials, double *__restrict__ macros,
             const FusedArgs a) {
     using L = Lat<Q>;
-    using T = typename S::T;
     using C = typename S::C;
     const Geo &g = a.g;
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -89,7 +88,7 @@ __global__ void __launch_bounds__(512)
     pull<Q, S, FORCE>(A, flags, g, a.sp, x, y, z, node, live, near, gv, fx, fy, fz);
 
     // ---- 2. Zou-He inlet ----
-    if (!g.xper && x == 0 && live) zou_he<Q, C>(gv, (C)uin[y + g.ny * z]);
+    if (!g.xper && x == 0 && live) velocity_inlet<Q, C>(gv, (C)uin[y + g.ny * z]);
 
     // ---- 3./4. moments, collision, partial bounce-back, store ----
     C sf = 0;

@@ -39,6 +39,7 @@ class LbmInfo(C.Structure):
         ("lattice", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("pitch", C.c_int),
         ("nodes", C.c_size_t), ("state_bytes", C.c_size_t), ("bytes_per_update", C.c_double),
         ("step", C.c_long), ("solid_nodes", C.c_int), ("boundary_nodes", C.c_int),
+        ("nodes_per_lane", C.c_int),
     ]
 
 

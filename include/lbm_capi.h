@@ -76,6 +76,7 @@ typedef struct lbm_info {
     double bytes_per_update;          /* algorithmic 2*Q*sizeof(storage) */
     long step;                        /* steps taken since initialise */
     int solid_nodes, boundary_nodes;
+    int nodes_per_lane;               /* x nodes per lane of the fused kernel (plain mode) */
 } lbm_info;
 
 typedef struct lbm_ctx lbm_ctx;

@@ -645,6 +645,46 @@ static void zou_he(const lattice *L, double *gv, double U)
     }
 }
 
+/* Regularised inlet (Latt & Chopard's regularisation of the Zou-He node; no reference
+ * counterpart): with S = sum g and P_ab = sum g e_a e_b - S/3 delta_ab of the Zou-He-completed
+ * populations,  g_i = w_i (S + 3 e_ix U + 4.5 Q_i:P),  Q_i = e_i e_i - delta/3.  The sums run
+ * in direction order and every product is written out, so the FP64 GPU path (built, like
+ * this file, without FMA contraction) reproduces it bit for bit. */
+static void regularize_inlet(const lattice *L, double *g, double U)
+{
+    double S = 0.0, Mxx = 0.0, Myy = 0.0, Mzz = 0.0, Mxy = 0.0, Mxz = 0.0, Myz = 0.0;
+    for (int k = 0; k < L->Q; k++) {
+        const int ex = L->e[k][0], ey = L->e[k][1], ez = L->e[k][2];
+        S = S + g[k];
+        if (ex != 0) Mxx = Mxx + g[k];
+        if (ey != 0) Myy = Myy + g[k];
+        if (ez != 0) Mzz = Mzz + g[k];
+        if (ex * ey > 0) Mxy = Mxy + g[k]; else if (ex * ey < 0) Mxy = Mxy - g[k];
+        if (ex * ez > 0) Mxz = Mxz + g[k]; else if (ex * ez < 0) Mxz = Mxz - g[k];
+        if (ey * ez > 0) Myz = Myz + g[k]; else if (ey * ez < 0) Myz = Myz - g[k];
+    }
+    const double S3 = S * (1.0 / 3.0);
+    const double Pxx = Mxx - S3, Pyy = Myy - S3, Pzz = Mzz - S3;
+    double tr = Pxx + Pyy;
+    if (L->D == 3) tr = tr + Pzz;
+    const double d = tr * (1.0 / 3.0);
+    const double U3 = 3.0 * U;
+    for (int i = 0; i < L->Q; i++) {
+        const int ex = L->e[i][0], ey = L->e[i][1], ez = L->e[i][2];
+        double q = 0.0, o = 0.0;
+        if (ex != 0) q = q + Pxx;
+        if (ey != 0) q = q + Pyy;
+        if (ez != 0) q = q + Pzz;
+        q = q - d;
+        if (ex * ey > 0) o = o + Mxy; else if (ex * ey < 0) o = o - Mxy;
+        if (ex * ez > 0) o = o + Mxz; else if (ex * ez < 0) o = o - Mxz;
+        if (ey * ez > 0) o = o + Myz; else if (ey * ez < 0) o = o - Myz;
+        double t = S;
+        if (ex > 0) t = S + U3; else if (ex < 0) t = S - U3;
+        g[i] = L->t[i] * (t + (4.5 * q + 9.0 * o));
+    }
+}
+
 void orc_plain_step(orc_plain *s, int nsteps)
 {
     const lattice *L = &s->L;
@@ -687,7 +727,7 @@ void orc_plain_step(orc_plain *s, int nsteps)
                 }
             }
         }
-        /* 2. outlet (zero-gradient) and Zou-He inlet */
+        /* 2. outlet (zero-gradient) and regularised Zou-He inlet */
         if (!s->xper) {
 #pragma omp parallel for
             for (long r = 0; r < (long)rows; r++) {
@@ -703,6 +743,7 @@ void orc_plain_step(orc_plain *s, int nsteps)
                     double gv[27];
                     for (int i = 0; i < Q; i++) gv[i] = s->g[i * N + d];
                     zou_he(L, gv, s->uin[r]);
+                    regularize_inlet(L, gv, s->uin[r]);
                     for (int i = 0; i < Q; i++) s->g[i * N + d] = gv[i];
                 }
             }
@@ -861,10 +902,29 @@ static void adj_local(const orc_plain *s, const double *G, const double *lam, do
             for (int d = 0; d < 3; d++) eL += (double)L->e[j][d] * Lu[d];
             m[j] = (1.0 - om) * (1.0 - sf) * l[j] + om * (Lp + eL) + sf * l[L->bb[j]] + alpha * 2.0 * sf * L->e[j][0];
         }
-        /* Zou-He transpose at inlet nodes */
+        /* inlet nodes: transpose of the regularisation g -> A g + 3 w e_x U with the explicit
+         * matrix A_ij = w_i (1 + 4.5 Q_i:Q_j), then the Zou-He transpose */
         const int x = (int)(a % nx);
         if (!s->xper && x == 0) {
             const size_t row = a / nx;
+            const int D = L->D;
+            double mr[27];
+            for (int j = 0; j < Q; j++) {
+                double acc = 0.0;
+                for (int i = 0; i < Q; i++) {
+                    double qq = 0.0;
+                    for (int p1 = 0; p1 < D; p1++)
+                        for (int p2 = 0; p2 < D; p2++) {
+                            const double qi = (double)(L->e[i][p1] * L->e[i][p2]) - (p1 == p2 ? 1.0 / 3.0 : 0.0);
+                            const double qj = (double)(L->e[j][p1] * L->e[j][p2]) - (p1 == p2 ? 1.0 / 3.0 : 0.0);
+                            qq += qi * qj;
+                        }
+                    acc += m[i] * L->t[i] * (1.0 + 4.5 * qq);
+                }
+                mr[j] = acc;
+            }
+            for (int i = 0; i < Q; i++) dJ_duin[row] += m[i] * 3.0 * L->t[i] * (double)L->e[i][0];
+            for (int j = 0; j < Q; j++) m[j] = mr[j];
             for (int i = 0; i < Q; i++) {
                 if (L->e[i][0] <= 0) continue;
                 const double mi = m[i];

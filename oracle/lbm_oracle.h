@@ -14,7 +14,9 @@
  *               D3Q27: 3D MPI+KOKKOS+MPIIO/lbm.cpp:36-417, lbm.hpp:146-219 (ghost 3)
  *               D2Q9 : MPI Kokkos/lbm.cpp:47-139,313-457, lbm.hpp:77-85     (ghost 2)
  *  "plain" -- the configuration physics (no reference counterpart): half-way bounce-back
- *             on walls and on the solid mask, Zou-He velocity inlet, zero-gradient
+ *             on walls and on the solid mask, Zou-He velocity inlet regularised at the
+ *             inlet node (Latt & Chopard's projection: plain Zou-He is unstable with BGK
+ *             at tau near 1/2), zero-gradient
  *             outlet, optional periodic axes, optional per-node partial bounce-back
  *             (solid fraction s), with the reference's BGK collision and equilibrium.
  *             Written in PUSH form (sources scatter), independent in structure from the

@@ -186,12 +186,27 @@ def test_adjoint_matches_cpu_adjoint(pkg, orc, Q, shape, prec, tol):
     (27, (14, 20, 64), dict(body=1, center=(20.0, 9.5, 6.5), r2=9.0, inlet=1)),
     (27, (10, 12, 256), dict(body=2, center=(120.0, 6.0, 0.0), r2=8.0, periodic=(False, True, False))),
     (9, (1, 30, 200), dict(body=1, center=(40.0, 14.5, 0.0), r2=25.0)),
+    (27, (8, 10, 256), dict(body=3, mask="edges")),
+    (9, (1, 16, 128), dict(body=3, mask="edges")),
 ])
 def test_fast_path_matches_per_node_path(pkg, orc, monkeypatch, prec, V, Q, shape, kw):
-    """The wave-level fast path (V nodes per lane) and the per-node path give bit-identical
-    populations, fields and forces."""
+    """The wave-level packed paths (V nodes per lane, with and without mask links) and the
+    per-node path give bit-identical populations, fields and forces.  The "edges" mask puts
+    solid nodes on the inlet and outlet lanes, against the walls and as isolated points, so
+    lanes hold both solid and fluid nodes."""
     L = pkg.lbm
     nz, ny, nx = shape
+    kw = dict(kw)
+    mask = None
+    if kw.pop("mask", None) == "edges":
+        mask = np.zeros(shape, dtype=np.uint8)
+        mask[:, 2:5, 0] = 1                  # inlet column
+        mask[:, ny // 2, 1] = 1
+        mask[:, 3:6, nx - 1] = 1             # outlet column
+        mask[:, ny - 4, nx - 2] = 1
+        mask[:, 0, 40:43] = 1                # against the y = 0 wall
+        mask[:, ny // 2, 75] = 1             # isolated node, odd x
+        mask[:, ny // 2 + 1, 100:104] = 1
     g = L.make_config(Q, nx, ny, nz, tau0=0.05, u0=0.05, precision=L.PRECISIONS[prec], bc_mode=L.BC_PLAIN,
                       perturb=1e-3, force_interval=3, **kw)
     monkeypatch.setenv("LBM_NODES_PER_LANE", str(V))
@@ -199,7 +214,10 @@ def test_fast_path_matches_per_node_path(pkg, orc, monkeypatch, prec, V, Q, shap
     for slow in ("0", "1"):
         monkeypatch.setenv("LBM_FORCE_SLOW_PATH", slow)
         print(f"[parity] fast/per-node {prec} V={V} Q={Q} {shape} slow={slow}", flush=True)
-        s = pkg.Solver(g).initialize()
+        s = pkg.Solver(g)
+        if mask is not None:
+            s.set_mask(mask)
+        s.initialize()
         s.step(25)
         out.append((s.populations(), s.fields(), s.forces()))
         s.close()

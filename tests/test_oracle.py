@@ -191,6 +191,82 @@ def test_zou_he_inlet_momentum_is_exact(orc, Q, shape):
     assert not np.isnan(g).any()
 
 
+def textbook_regularize(orc, Q, g, U):
+    """Latt & Chopard's regularisation from its definition: g_i = feq_i(p, U) +
+    w_i / (2 cs^4) (e_i e_i - cs^2 I) : Pi_neq, Pi_neq = sum_k e_k e_k (g_k - feq_k), cs^2 = 1/3,
+    p = sum g / 3 (shifted incompressible model).  g: (Q, n) columns of populations."""
+    e, w, _ = orc.lattice(Q)
+    D = 2 if Q == 9 else 3
+    e = e[:, :D].astype(float)
+    out = np.empty_like(g)
+    for c in range(g.shape[1]):
+        p = g[:, c].sum() / 3.0
+        fe = orc.feq(Q, p, U[c])
+        pi = np.einsum("k,ka,kb->ab", g[:, c] - fe, e, e)
+        qi = np.einsum("ia,ib->iab", e, e) - np.eye(D) / 3.0
+        out[:, c] = fe + w * 4.5 * np.einsum("iab,ab->i", qi, pi)
+    return out
+
+
+def test_regularized_inlet_matches_textbook_d2q9(orc):
+    """Zou & He (1997) D2Q9 velocity inlet (rho u -> u for the incompressible model) followed by
+    the textbook regularisation, computed here from the pulled populations of a random state,
+    equals the oracle's inlet column after one step."""
+    nx, ny, U = 6, 5, 0.05
+    s = orc.PlainState(orc.make_cfg(9, nx, ny, 1, tau0=0.1, u0=U, periodic=(False, True, False)))
+    rng = np.random.default_rng(11)
+    H = np.stack([orc.feq(9, 0.0, U) for _ in range(nx * ny)], 1).reshape(9, 1, ny, nx)
+    H = H + 1e-3 * rng.standard_normal(H.shape)
+    s.set_populations(H)
+    s.step(1)
+    out = s.last_g()[:, 0, :, 0]
+    e, w, _ = orc.lattice(9)
+    g = np.zeros((9, ny))
+    for i in range(9):
+        ex, ey = e[i, 0], e[i, 1]
+        if ex <= 0:   # knowns: pulled from (0 - ex, y - ey), periodic in y
+            g[i] = np.roll(H[i, 0, :, -ex], ey)
+    # classic numbering 1:(1,0) 2:(0,1) 3:(-1,0) 4:(0,-1) 5:(1,1) 6:(-1,1) 7:(-1,-1) 8:(1,-1)
+    assert [tuple(e[i, :2]) for i in range(9)] == [(0, 0), (1, 0), (0, 1), (-1, 0), (0, -1), (1, 1), (-1, 1),
+                                                  (-1, -1), (1, -1)]
+    g[1] = g[3] + 2.0 / 3.0 * U
+    g[5] = g[7] - 0.5 * (g[2] - g[4]) + U / 6.0
+    g[8] = g[6] + 0.5 * (g[2] - g[4]) + U / 6.0
+    ref = textbook_regularize(orc, 9, g, np.full(ny, U))
+    assert np.abs(out - ref).max() < 1e-15
+
+
+@pytest.mark.parametrize("inlet", [0, 1])
+def test_regularized_inlet_properties_d3q27(orc, inlet):
+    """D3Q27: the inlet column carries exactly the prescribed momentum and is a fixed point of
+    the textbook regularisation (its non-hydrodynamic moments are those of feq + Pi_neq)."""
+    nz, ny, nx = 5, 6, 7
+    s = orc.PlainState(orc.make_cfg(27, nx, ny, nz, tau0=0.1, u0=0.05, inlet=inlet, perturb=1e-3))
+    rng = np.random.default_rng(3)
+    s.set_populations(s.populations() + 1e-3 * rng.standard_normal((27, nz, ny, nx)))
+    s.step(1)
+    out = s.last_g()[:, :, :, 0].reshape(27, -1)
+    U = s.inlet().reshape(-1)
+    e, _, _ = orc.lattice(27)
+    mom = e.astype(float).T @ out
+    assert np.abs(mom[0] - U).max() < 1e-16 and np.abs(mom[1:]).max() < 1e-16
+    assert np.abs(textbook_regularize(orc, 27, out, U) - out).max() < 1e-15
+
+
+@pytest.mark.parametrize("Q,shape,tau0,steps", [(9, (1, 20, 100), 0.03, 2000), (27, (6, 6, 40), 0.018, 1500)])
+def test_velocity_inlet_is_stable_near_tau_half(orc, Q, shape, tau0, steps):
+    """Uniform flow with a 1e-3 perturbation through the inlet and outlet at tau = 0.53 (D2Q9,
+    the Re 100 cylinder workload) and 0.518 (D3Q27, the Re 200 sphere workload) decays.  Plain
+    Zou-He (no regularisation) diverges from the inlet column within ~1000 steps here."""
+    nz, ny, nx = shape
+    per = (False, True, Q == 27)
+    s = orc.PlainState(orc.make_cfg(Q, nx, ny, nz, tau0=tau0, u0=0.05, periodic=per, perturb=1e-3))
+    s.step(steps)
+    u = s.fields()[1]
+    assert np.isfinite(u).all()
+    assert np.abs(u - 0.05).max() < 1e-4
+
+
 @pytest.mark.parametrize("Q", [9, 27])
 def test_shear_mode_between_walls_decays_with_wall_half_node_outside(orc, Q):
     """u_x = U sin(pi (y + 1/2)/n) between no-slip walls at y = -1/2 and y = n - 1/2 decays as
