@@ -4,7 +4,7 @@ plus the adjoint/forward time ratio.  Prints ONE JSON line.
 
 Workload (the largest single-GPU configuration): D3Q27 BGK, 512x256x256 nodes (33.5 M),
 z-spanning cylinder D=40 at (128,128), Re=300, u=0.05, half-way bounce-back on the body and the
-four side walls, Zou-He inlet, zero-gradient outlet, drag/lift every 10 steps.  Synthetic
+four side walls, Zou-He inlet (regularised), zero-gradient outlet, drag/lift every 10 steps.  Synthetic
 initial state (seeded perturbation); the populations are resident in HBM before timing starts.
 
 A "step" is one fused lattice update of the whole domain.  MLUPS = nodes * steps / seconds over
@@ -113,6 +113,52 @@ def adjoint_ratio(pkg, steps=500):
             "forward_ms": round(tf, 3), "adjoint_ms": round(ta, 3), "ratio": round(ta / tf, 3)}
 
 
+def accuracy(pkg, steps, early, window_start=None):
+    """Storage formats against FP64 on the benchmark workload: mean drag and RMS lift
+    coefficients over steps [window_start, steps] (default: the second half), the Strouhal
+    number of the lift signal band-passed to St in [0.05, 0.5] (which removes the channel's
+    transverse acoustic mode, period 2 ny / c_s = 887 steps, St 0.9), and the relative L2 error
+    of the velocity field after `early` steps (before the unsteady wakes drift apart in phase).
+    The FP64 run is the oracle's algorithm on the GPU (bit-identical to the CPU oracle on the
+    parity cases; the CPU oracle itself would need hours at this size)."""
+    ws = steps // 2 if window_start is None else window_start
+    import numpy as np
+    L = pkg.lbm
+    out, ref = {}, None
+    for name, prec in (("fp64", L.FP64), ("fp32", L.FP32), ("fp16s", L.FP16S), ("bf16s", L.BF16S)):
+        W = pkg.configs.cfg2_zcyl3d(precision=prec, force_interval=10)
+        s = pkg.Solver(W.cfg).initialize()
+        s.step(early)
+        _, u, v, w = s.fields()
+        t0 = time.perf_counter()
+        s.step(steps - early)
+        s.sync()
+        dt = time.perf_counter() - t0
+        F = s.forces()
+        s.close()
+        sel = F[:, 0] > ws
+        cd = W.cd_scale() * F[sel, 1]
+        cl = W.cd_scale() * F[sel, 2]
+        rec = {"cd_mean": float(cd.mean()), "cl_rms": float(np.sqrt(np.mean((cl - cl.mean()) ** 2))),
+               "strouhal": float(pkg.configs.strouhal(cl, 10.0, W.D, W.u0, band=(0.05, 0.5))),
+               "seconds": round(dt, 2)}
+        if ref is None:
+            ref = (rec, u, v, w)
+        else:
+            r, ur, vr, wr = ref
+            num = np.sqrt(np.sum((u - ur) ** 2) + np.sum((v - vr) ** 2) + np.sum((w - wr) ** 2))
+            den = np.sqrt(np.sum(ur ** 2) + np.sum(vr ** 2) + np.sum(wr ** 2))
+            rec["vel_l2_rel_err_at_early"] = float(num / den)
+            for k in ("cd_mean", "cl_rms", "strouhal"):
+                rec[k + "_rel_err"] = float(abs(rec[k] - r[k]) / abs(r[k]))
+        del u, v, w
+        out[name] = rec
+        log(f"accuracy {name}: {rec}")
+    out["workload"] = f"cfg2 512x256x256 Re 300, {steps} steps, statistics over steps {ws}-{steps}, " \
+                      f"velocity compared at step {early}"
+    return out
+
+
 def copy_calibration(n_bytes=1 << 30, reps=3):
     """A plain device-to-device copy of known size (read n_bytes + write n_bytes) through the
     HIP runtime (its copy kernel): that dispatch's FETCH_SIZE / WRITE_SIZE counters give the
@@ -151,6 +197,10 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--accuracy", type=int, default=0, metavar="STEPS",
+                    help="instead of timing, compare FP32/FP16/BF16 storage with FP64 over STEPS steps")
+    ap.add_argument("--accuracy-early", type=int, default=2000)
+    ap.add_argument("--accuracy-window", type=int, default=None, help="first step of the statistics window")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the whole domain runs on a single MI355X; "
@@ -159,6 +209,9 @@ def main():
     pkg = load_package()
     L = pkg.lbm
     K, Wm = a.steps, a.warmup
+    if a.accuracy:
+        print(json.dumps({"accuracy": accuracy(pkg, a.accuracy, a.accuracy_early, a.accuracy_window)}), flush=True)
+        return
 
     log(f"fp16s leg: {K} steps after {Wm} warm-up")
     s16, W16, wall16, info16 = run_leg(pkg, L.FP16S, K, Wm, 10)

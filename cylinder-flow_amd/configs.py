@@ -3,7 +3,8 @@
 Lattice units throughout.  nu = u*D/Re and tau0 = 3*nu (the solver relaxes with
 tau = tau0 + 0.5, as the reference does: 3D MPI+KOKKOS+MPIIO/System.cpp:18-19,
 lbm.cpp:269).  Every workload uses the plain boundary mode: half-way bounce-back on the
-body and the side walls, Zou-He velocity inlet, zero-gradient outlet, and an initial state
+body and the side walls, Zou-He velocity inlet (regularised at the inlet node),
+zero-gradient outlet, and an initial state
 p = 0, u = (u_in + 1e-3 xi, 0, 0) with xi uniform in [-1, 1) from the seeded counter RNG.
 """
 from __future__ import annotations
@@ -117,12 +118,34 @@ ALL = {"cfg0": cfg0_cylinder2d, "cfg1": cfg1_sphere3d, "cfg2": cfg2_zcyl3d, "cfg
        "cfg4": cfg4_adjoint3d}
 
 
-def strouhal(cl: np.ndarray, dt_steps: float, D: float, u: float) -> float:
-    """Shedding frequency from upward zero crossings of the lift signal (mean removed)."""
-    x = cl - cl.mean()
-    up = np.nonzero((x[:-1] < 0) & (x[1:] >= 0))[0]
-    if len(up) < 3:
+def strouhal(cl: np.ndarray, dt_steps: float, D: float, u: float, min_periods: int = 3,
+             max_cv: float = 0.2, band: tuple | None = None) -> float:
+    """Shedding frequency St = D / (T u) from the upward crossings of the lift signal (mean
+    removed) through +h after it was below -h, h = std/4 (hysteresis, so noise near zero does
+    not add crossings); crossings are interpolated linearly.  NaN unless the signal is
+    periodic: at least `min_periods` full periods whose lengths vary by less than `max_cv`
+    (coefficient of variation), so a wake that has not started to shed gives NaN.
+    band = (St_lo, St_hi) first keeps only those frequencies (FFT band-pass), e.g. to remove a
+    channel's transverse acoustic mode (period 2 ny / c_s steps) from the lift."""
+    x = np.asarray(cl, dtype=np.float64) - np.mean(cl)
+    if band is not None:
+        X = np.fft.rfft(x)
+        st = np.fft.rfftfreq(len(x), d=dt_steps) * D / u
+        X[(st < band[0]) | (st > band[1])] = 0.0
+        x = np.fft.irfft(X, n=len(x))
+    h = 0.25 * x.std()
+    if not h > 0:
         return float("nan")
-    t = up + (-x[up] / (x[up + 1] - x[up]))  # linear interpolation of the crossing
-    period = np.mean(np.diff(t)) * dt_steps
-    return D / (period * u)
+    t, low = [], False
+    for i in range(1, len(x)):
+        if x[i] < -h:
+            low = True
+        elif low and x[i] >= h and x[i - 1] < h:
+            t.append(i - 1 + (h - x[i - 1]) / (x[i] - x[i - 1]))
+            low = False
+    if len(t) < min_periods + 1:
+        return float("nan")
+    periods = np.diff(t) * dt_steps
+    if periods.std() > max_cv * periods.mean():
+        return float("nan")
+    return D / (periods.mean() * u)

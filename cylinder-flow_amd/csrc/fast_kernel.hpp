@@ -1,16 +1,22 @@
 // fast_kernel.hpp -- fused pull step for FP32 / FP16 / BF16 storage, V nodes per lane.
 //
 // Same boundary rules and arithmetic as the per-node path (fused_common.hpp), organised for
-// gfx950 memory throughput:
-//  * every lane owns V consecutive x nodes, so each of the 27 pulls and 27 stores is one
-//    V*sizeof(storage)-byte access (a dword for two FP16 nodes) instead of V narrow ones;
-//  * a wave covers 64*V nodes of one x row.  A per-(row, wave) class, computed once from the
-//    mask, picks its path: packed (no solid node or mask link), packed with link words
-//    (solid nodes or mask links: bounce-back values and momentum exchange patched in per
-//    node), or per-node (padding columns, a periodic x edge, a solid fraction);
+// gfx950 memory throughput.  One step is two launches on the solver stream:
+//  * k_fast_v runs the packed path on every lane of every row: each lane owns V consecutive x
+//    nodes, so each of the 27 pulls and 27 stores is one V*sizeof(storage)-byte access (a
+//    dword for two FP16 nodes); the source row of a direction is a scalar and only three lane
+//    offsets exist (x-1, x, x+1).  It applies walls, periodic y/z, the inlet and the outlet,
+//    but knows nothing of the mask, so nodes next to the body get provisional values and
+//    solid nodes get values that no fluid node ever reads;
+//  * k_fix_v then recomputes, with the per-node pull, every node the packed path gets wrong
+//    (a compact list built once from the mask: neighbours of solid nodes, nodes with a solid
+//    fraction, the periodic x edges, the last node of a lane that reaches into the padding)
+//    and writes the drag/lift partials of the nodes that carry force.
+// The rare boundary work thus costs the fast kernel no registers: it stays at the occupancy
+// of the packed path alone.  Both kernels use collide(), so a node's result does not depend
+// on the path it took.
 //  * feq_i and feq_opp(i) share w_i (3p - 1.5|u|^2 + 4.5 (e.u)^2) and differ by the sign of
 //    3 w_i e.u, so the equilibrium is evaluated per opposite pair.
-// Both paths use collide(), so a node's result does not depend on the path it took.
 #pragma once
 #include "fused_common.hpp"
 
@@ -88,8 +94,6 @@ template <int Bytes> struct UInt;
 template <> struct UInt<2> { using type = uint16_t; };
 template <> struct UInt<4> { using type = uint32_t; };
 template <> struct UInt<8> { using type = uint64_t; };
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-template <> struct UInt<16> { using type = u32x4; };
 
 template <class T, int V>
 __device__ __forceinline__ Pack<T, V> load_pack(const char *p) {
@@ -130,43 +134,23 @@ inline PlaneTab plane_tab(const Geo &g) {
     return t;
 }
 
-// Packed path of one lane: V consecutive nodes of a wave with no padding column and no
-// periodic x edge.  The source row of each direction is a scalar (walls and periodic y/z are
-// decided per block) and only three lane offsets exist (x-1, x, x+1), so a pull is one
-// V-wide load from a scalar row pointer plus a 32-bit lane offset.  The inlet lane (x = 0)
-// and the outlet lane (x = nx-1) load their row aligned and shift one value.
-// NEAR (waves touching the mask): the lane's link words pick, per node and direction, the
-// bounced value h_opp(i)(node), loaded V-wide from the node's own row when any node of the
-// lane needs it; the momentum exchange of those links is accumulated node by node in
-// direction order, as the per-node pull does, and solid nodes are neither updated nor stored.
-template <int Q, class S, int V, bool FORCE, bool MACROS, bool NEAR>
+// Packed path of one lane: V consecutive nodes of one x row.  Interior rows (no wall, no
+// periodic wrap in y/z) address direction i at the scalar Arow + tab.src[i]; boundary rows pick
+// the source row (or the wall bounce-back row) per direction.  The inlet lane (x = 0) and the
+// lane holding the outlet node (x = nx-1) load their row aligned and shift one value; a lane
+// past the outlet (padding) clamps the same way, so every access stays inside its row.
+template <int Q, class S, int V, bool MACROS>
 __device__ __forceinline__ void packed_nodes(const typename S::T *__restrict__ A, char *__restrict__ Brow,
-                                             const uint32_t *__restrict__ links, const double *__restrict__ uin,
-                                             double *__restrict__ macros, const FusedArgs &a, const PlaneTab &tab,
-                                             int y, int z, int x0, size_t rowoff, double &fx, double &fy,
-                                             double &fz) {
+                                             const double *__restrict__ uin, double *__restrict__ macros,
+                                             const FusedArgs &a, const PlaneTab &tab, int y, int z, int x0,
+                                             size_t rowoff) {
     using L = Lat<Q>;
     using T = typename S::T;
     const Geo &g = a.g;
-    const bool first = x0 == 0, last = x0 + V == g.nx;
+    const bool first = x0 == 0, last = x0 + V >= g.nx;
     const uint32_t b0 = (uint32_t)x0 * sizeof(T);
     const uint32_t bP = first ? b0 : b0 - (uint32_t)sizeof(T);   // e_x = +1 pulls x-1
     const uint32_t bM = last ? b0 : b0 + (uint32_t)sizeof(T);    // e_x = -1 pulls x+1
-    uint32_t lk[V];
-    uint32_t anyl = 0;
-    bool all_live = true;
-    if constexpr (NEAR) {
-        const Pack<uint32_t, V> lp = load_pack<uint32_t, V>((const char *)(links + rowoff) + (size_t)x0 * 4);
-        static_for<0, V>([&](auto K) {
-            lk[K] = lp.e[K];
-            anyl |= lk[K];
-            all_live = all_live && !(lk[K] & LINK_SOLID);
-        });
-    }
-    // every source load first (unconditional, valid addresses), then the bounced ones.
-    // Interior rows (no wall, no periodic wrap in y/z) address direction i at the scalar
-    // Arow + tab.src[i]; boundary rows pick the source row (or the wall bounce-back row) per
-    // direction.
     const char *const Arow = (const char *)A + rowoff * sizeof(T);
     const bool interior = (unsigned)(y - 1) < (unsigned)(g.ny - 2) &&
                           (L::D == 2 || (unsigned)(z - 1) < (unsigned)(g.nz - 2));
@@ -207,14 +191,6 @@ __device__ __forceinline__ void packed_nodes(const typename S::T *__restrict__ A
             bbw[i] = bb;
         });
     }
-    Pack<T, V> rb[Q];
-    if constexpr (NEAR) {
-        static_for<1, Q>([&](auto I) {
-            constexpr int i = I;
-            constexpr int o = L::opp(i);
-            if (anyl & (1u << i)) rb[i] = load_pack<T, V>(Arow + tab.dst[o] + b0);
-        });
-    }
     float gv[V][Q];
     static_for<0, Q>([&](auto I) {
         constexpr int i = I;
@@ -223,155 +199,121 @@ __device__ __forceinline__ void packed_nodes(const typename S::T *__restrict__ A
         const float off = bb ? a.sp.off[o] : a.sp.off[i];
         float f[V];
         static_for<0, V>([&](auto K) { f[K] = S::template decode<i, L>(r[i].e[K], off); });
-        if constexpr (ex > 0 && V > 1) {   // inlet lane: node k pulls element k-1 (node 0: Zou-He)
+        if constexpr (ex > 0 && V > 1) {   // inlet lane: node k pulls element k-1 (node 0: inlet)
             if (!bb) static_for<1, V>([&](auto K) { f[V - K] = first ? f[V - K - 1] : f[V - K]; });
         }
         if constexpr (ex < 0 && V > 1) {   // outlet lane: node k pulls k+1, node V-1 clamps to itself
             if (!bb) static_for<0, V - 1>([&](auto K) { f[K] = last ? f[K + 1] : f[K]; });
         }
-        if constexpr (NEAR && i > 0) {
-            static_for<0, V>([&](auto K) {
-                if (lk[K] & (1u << i)) f[K] = S::template decode<i, L>(rb[i].e[K], a.sp.off[o]);
-            });
-        }
         static_for<0, V>([&](auto K) { gv[K][i] = f[K]; });
     });
-    if constexpr (NEAR) {
-        static_for<0, V>([&](auto K) {
-            if (lk[K] & LINK_SOLID) static_for<0, Q>([&](auto I) { gv[K][I] = 0.f; });
-            if constexpr (FORCE) {
-                // momentum exchange of the bounced population h_opp(i): 2 h e_opp(i)
-                static_for<1, Q>([&](auto I) {
-                    constexpr int i = I;
-                    constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i);
-                    if (lk[K] & (1u << i)) {
-                        const double m = 2.0 * (double)gv[K][i];
-                        if constexpr (ex != 0) fx -= ex * m;
-                        if constexpr (ey != 0) fy -= ey * m;
-                        if constexpr (ez != 0) fz -= ez * m;
-                    }
-                });
-            }
-        });
-    }
-    if (first && (!NEAR || !(lk[0] & LINK_SOLID))) velocity_inlet<Q, float>(gv[0], (float)uin[y + g.ny * z]);
+    if (first) velocity_inlet<Q, float>(gv[0], (float)uin[y + g.ny * z]);
     Pack<T, V> out[Q];
     static_for<0, V>([&](auto K) {
         float c[Q], s, u, v, w;
+        double fx = 0.0, fy = 0.0, fz = 0.0;
         collide<Q, false, false>(gv[K], 0.f, a.omega, c, s, u, v, w, fx, fy, fz);
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
             out[i].e[K] = S::template enc<i, L>(c[i], a.sp);
         });
         if constexpr (MACROS) {
-            const bool live = !NEAR || !(lk[K] & LINK_SOLID);
             double *m = macros + rowoff + x0 + K;
-            m[0] = live ? (double)s * (1.0 / 3.0) : 0.0;
-            m[g.plane] = live ? (double)u : 0.0;
-            m[2 * g.plane] = live ? (double)v : 0.0;
-            m[3 * g.plane] = live ? (double)w : 0.0;
+            m[0] = (double)s * (1.0 / 3.0);
+            m[g.plane] = (double)u;
+            m[2 * g.plane] = (double)v;
+            m[3 * g.plane] = (double)w;
         }
     });
-    if (!NEAR || all_live) {
-        static_for<0, Q>([&](auto I) {
-            constexpr int i = I;
-            store_pack<T, V>(Brow + tab.dst[i] + b0, out[i]);
-        });
-    } else {
-        static_for<0, V>([&](auto K) {
-            if (!(lk[K] & LINK_SOLID)) {
-                static_for<0, Q>([&](auto I) {
-                    constexpr int i = I;
-                    __builtin_nontemporal_store(out[i].e[K], (T *)(Brow + tab.dst[i] + b0 + K * sizeof(T)));
-                });
-            }
-        });
-    }
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = I;
+        store_pack<T, V>(Brow + tab.dst[i] + b0, out[i]);
+    });
 }
 
-template <int Q, class S, int V, bool FORCE, bool MACROS, bool SFRAC>
+// Every lane of every row (256-lane blocks along x, one row per (blockIdx.y, blockIdx.z)):
+// lanes past the padded row return; nothing else branches on the node, and there is no
+// barrier.
+template <int Q, class S, int V, bool MACROS>
 __global__ void __launch_bounds__(256)
-    k_fused_v(const typename S::T *__restrict__ A, typename S::T *__restrict__ B, const uint8_t *__restrict__ flags,
-              const uint8_t *__restrict__ segcls, int nseg, const uint32_t *__restrict__ links,
-              const float *__restrict__ sfrac, const double *__restrict__ uin, const int *__restrict__ block_slot,
-              double *__restrict__ partials, double *__restrict__ macros, const FusedArgs a, const PlaneTab tab) {
+    k_fast_v(const typename S::T *__restrict__ A, typename S::T *__restrict__ B, const double *__restrict__ uin,
+             double *__restrict__ macros, const FusedArgs a, const PlaneTab tab) {
+    const Geo &g = a.g;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;   // lane index within the x row
+    if (t * V >= g.nxp) return;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const size_t rowoff = (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
+    packed_nodes<Q, S, V, MACROS>(A, (char *)(B + rowoff), uin, macros, a, tab, y, z, t * V, rowoff);
+}
+
+// The per-node recomputation of the listed nodes (padded-layout indices, row-major order),
+// after k_fast_v on the same stream: the general pull of fused_common.hpp (walls, inlet,
+// outlet, periodic axes, mask bounce-back), the velocity inlet, collision with partial
+// bounce-back, single-value stores.  FORCE writes each node's momentum exchange (and partial
+// bounce-back force) to partials[3 * slot[j]] when slot[j] >= 0; the slots are numbered in
+// node order, so the later fixed-order sum does not depend on which nodes the list holds.
+template <int Q, class S, bool FORCE, bool MACROS, bool SFRAC>
+__global__ void __launch_bounds__(256)
+    k_fix_v(const typename S::T *__restrict__ A, typename S::T *__restrict__ B, const uint8_t *__restrict__ flags,
+            const uint32_t *__restrict__ list, int nlist, const int *__restrict__ slot,
+            const float *__restrict__ sfrac, const double *__restrict__ uin, double *__restrict__ partials,
+            double *__restrict__ macros, const FusedArgs a, const PlaneTab tab) {
     using L = Lat<Q>;
     using T = typename S::T;
     const Geo &g = a.g;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;   // lane index within the x row
-    const int y = blockIdx.y, z = blockIdx.z;
-    const int x0 = t * V;
-    const size_t rowoff = (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
-    char *const Brow = (char *)(B + rowoff);                 // this block's row, direction 0
-    const int seg = __builtin_amdgcn_readfirstlane(t >> 6);
-    const int cls = segcls[(size_t)(y + g.ny * z) * nseg + seg];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nlist) return;
+    const uint32_t node = list[j];
+    const uint32_t row = node / (uint32_t)g.nxp;
+    const int x = (int)(node - row * (uint32_t)g.nxp);
+    const int y = (int)(row % (uint32_t)g.ny), z = (int)(row / (uint32_t)g.ny);
+    const uint8_t fl = flags[node];
     double fx = 0.0, fy = 0.0, fz = 0.0;
-
-    if (cls == WAVE_FAST) {
-        packed_nodes<Q, S, V, FORCE, MACROS, false>(A, Brow, links, uin, macros, a, tab, y, z, x0, rowoff, fx, fy, fz);
-    } else if (cls == WAVE_NEAR) {
-        packed_nodes<Q, S, V, FORCE, MACROS, true>(A, Brow, links, uin, macros, a, tab, y, z, x0, rowoff, fx, fy, fz);
-    } else {
-        // per-node path: the general pull of fused_common.hpp, one node at a time
-        for (int k = 0; k < V; k++) {
-            const int x = x0 + k;
-            const bool active = x < g.nx;
-            const uint32_t node = (uint32_t)(rowoff + (active ? x : 0));   // valid address for idle lanes
-            const uint8_t fl = active ? flags[node] : (uint8_t)FL_SOLID;
-            const bool live = fl != FL_SOLID;
-            float gv[Q], c[Q], s, u, v, w;
-            pull<Q, S, FORCE>(A, flags, g, a.sp, x, y, z, node, live, fl == FL_NEAR, gv, fx, fy, fz);
-            if (!g.xper && x == 0 && live) velocity_inlet<Q, float>(gv, (float)uin[y + g.ny * z]);
-            float sf = 0.f;
-            if constexpr (SFRAC) sf = live ? sfrac[node] : 0.f;
-            collide<Q, FORCE, SFRAC>(gv, sf, a.omega, c, s, u, v, w, fx, fy, fz);
-            if (live) {
-                static_for<0, Q>([&](auto I) {
-                    constexpr int i = I;
-                    __builtin_nontemporal_store(S::template enc<i, L>(c[i], a.sp),
-                                                (T *)(Brow + tab.dst[i] + (uint32_t)x * sizeof(T)));
-                });
-            }
-            if constexpr (MACROS) {
-                if (active) {
-                    double *m = macros + rowoff + x;
-                    m[0] = live ? (double)s * (1.0 / 3.0) : 0.0;
-                    m[g.plane] = live ? (double)u : 0.0;
-                    m[2 * g.plane] = live ? (double)v : 0.0;
-                    m[3 * g.plane] = live ? (double)w : 0.0;
-                }
-            }
-        }
+    float gv[Q], c[Q], s, u, v, w;
+    pull<Q, S, FORCE>(A, flags, g, a.sp, x, y, z, node, true, fl == FL_NEAR, gv, fx, fy, fz);
+    if (!g.xper && x == 0) velocity_inlet<Q, float>(gv, (float)uin[y + g.ny * z]);
+    float sf = 0.f;
+    if constexpr (SFRAC) sf = sfrac[node];
+    collide<Q, FORCE, SFRAC>(gv, sf, a.omega, c, s, u, v, w, fx, fy, fz);
+    char *const Bn = (char *)B + (size_t)node * sizeof(T);
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = I;
+        __builtin_nontemporal_store(S::template enc<i, L>(c[i], a.sp), (T *)(Bn + tab.dst[i]));
+    });
+    if constexpr (MACROS) {
+        double *m = macros + node;
+        m[0] = (double)s * (1.0 / 3.0);
+        m[g.plane] = (double)u;
+        m[2 * g.plane] = (double)v;
+        m[3 * g.plane] = (double)w;
     }
     if constexpr (FORCE) {
-        const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-        const int slot = block_slot[bid];
-        if (slot >= 0) {
-            __shared__ double red[3 * 4];
-            block_sum3<4>(fx, fy, fz, red);
-            if (threadIdx.x == 0) {
-                partials[3 * (size_t)slot] = fx;
-                partials[3 * (size_t)slot + 1] = fy;
-                partials[3 * (size_t)slot + 2] = fz;
-            }
+        const int k = slot[j];
+        if (k >= 0) {
+            partials[3 * (size_t)k] = fx;
+            partials[3 * (size_t)k + 1] = fy;
+            partials[3 * (size_t)k + 2] = fz;
         }
     }
 }
 
+// One step: the packed path everywhere, then the listed nodes.
 template <int Q, class S, int V, bool FORCE, bool MACROS>
 static hipError_t fast_launch(const StepBufs &b, const FusedArgs &a, hipStream_t s) {
     using T = typename S::T;
     const Shape sh = lane_shape(a.g, V);
-    const dim3 grid(sh.gx, a.g.ny, a.g.nz);
     const PlaneTab tab = plane_tab<Q, T>(a.g);
+    hipLaunchKernelGGL((k_fast_v<Q, S, V, MACROS>), dim3(sh.gx, a.g.ny, a.g.nz), dim3(sh.bx), 0, s, (const T *)b.A,
+                       (T *)b.B, b.uin, b.macros, a, tab);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || b.nfix == 0) return e;
+    const dim3 grid((unsigned)((b.nfix + 255) / 256));
     if (b.sfrac)
-        hipLaunchKernelGGL((k_fused_v<Q, S, V, FORCE, MACROS, true>), grid, dim3(sh.bx), 0, s, (const T *)b.A, (T *)b.B,
-                           b.flags, b.segcls, b.nseg, b.links, (const float *)b.sfrac, b.uin, b.block_slot, b.partials,
-                           b.macros, a, tab);
+        hipLaunchKernelGGL((k_fix_v<Q, S, FORCE, MACROS, true>), grid, dim3(256), 0, s, (const T *)b.A, (T *)b.B, b.flags,
+                           b.fix, b.nfix, b.block_slot, (const float *)b.sfrac, b.uin, b.partials, b.macros, a, tab);
     else
-        hipLaunchKernelGGL((k_fused_v<Q, S, V, FORCE, MACROS, false>), grid, dim3(sh.bx), 0, s, (const T *)b.A, (T *)b.B,
-                           b.flags, b.segcls, b.nseg, b.links, (const float *)nullptr, b.uin, b.block_slot, b.partials,
-                           b.macros, a, tab);
+        hipLaunchKernelGGL((k_fix_v<Q, S, FORCE, MACROS, false>), grid, dim3(256), 0, s, (const T *)b.A, (T *)b.B, b.flags,
+                           b.fix, b.nfix, b.block_slot, (const float *)nullptr, b.uin, b.partials, b.macros, a, tab);
     return hipGetLastError();
 }
 

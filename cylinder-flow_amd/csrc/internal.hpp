@@ -32,15 +32,6 @@ struct FusedArgs {
 // node flags
 enum : uint8_t { FL_FLUID = 0, FL_SOLID = 1, FL_NEAR = 2 /* fluid with a solid neighbour */ };
 
-// wave classes of the FP32/16-bit fused kernel, one per (row, 64-lane wave):
-//   WAVE_FAST  every node fluid with fluid sources, no padding, x not periodic at this wave
-//   WAVE_NEAR  as WAVE_FAST but with solid nodes or mask links: packed loads plus a per-node
-//              link word (bit i: direction i pulls from a solid node; LINK_SOLID: the node
-//              itself is solid and is neither updated nor stored)
-//   WAVE_NODE  per-node path (padding columns, periodic x edge, solid fraction)
-enum : uint8_t { WAVE_FAST = 0, WAVE_NEAR = 1, WAVE_NODE = 2 };
-constexpr uint32_t LINK_SOLID = 1u << 31;
-
 // launch shape of every per-node kernel: blocks of `bx` threads along x, `gx` per row
 struct Shape {
     int bx, gx;
@@ -57,12 +48,11 @@ struct StepBufs {
     const void *A;              // populations in (storage type)
     void *B;                    // populations out
     const uint8_t *flags;       // FL_* per node (padded layout)
-    const uint8_t *segcls;      // per (row, 64-lane wave): a WAVE_* class (FP32/16-bit kernels)
-    int nseg;                   // waves per row
-    const uint32_t *links;      // per node: LINK_* bits (null when no wave is WAVE_NEAR)
+    const uint32_t *fix;        // FP32/16-bit: nodes the packed path gets wrong (padded indices)
+    int nfix;
     const void *sfrac;          // solid fraction (FP64: double, else float) or null
     const double *uin;          // inlet velocity per (y, z)
-    const int *block_slot;      // force partial slot per block (-1: none)
+    const int *block_slot;      // force partial slot per block (FP64) or per listed node (-1: none)
     double *partials;
     double *macros;             // p, u, v, w planes (written when macros_on)
 };
@@ -71,13 +61,17 @@ hipError_t plain_step(int Q, int prec, int V, const StepBufs &b, bool force, boo
 hipError_t fast_step_f32(int Q, int V, const StepBufs &b, bool force, bool mac, const FusedArgs &a, hipStream_t s);
 hipError_t fast_step_f16(int Q, int V, const StepBufs &b, bool force, bool mac, const FusedArgs &a, hipStream_t s);
 hipError_t fast_step_bf16(int Q, int V, const StepBufs &b, bool force, bool mac, const FusedArgs &a, hipStream_t s);
-hipError_t force_reduce(const double *partials, int nslots, double *hist, int rec, long step, hipStream_t s);
+// sum of the nslots force partials into hist[rec] (chunks: force_chunks(nslots) * 3 doubles)
+int force_chunks(int nslots);
+hipError_t force_reduce(const double *partials, int nslots, double *chunks, double *hist, int rec, long step,
+                        hipStream_t s);
 // FP64 <-> storage conversion of one direction plane (physical nodes, x fastest)
 hipError_t plain_decode(int Q, int prec, const void *A, double *out, const uint8_t *flags, int dir,
                         const FusedArgs &a, hipStream_t s);
 hipError_t plain_encode(int Q, int prec, void *A, const double *in, const uint8_t *flags, int dir,
                         const FusedArgs &a, hipStream_t s);
-hipError_t macros_extract(const double *macros, double *out, int k, const Geo &g, hipStream_t s);
+// moment plane k (p, u, v, w) at the physical nodes; 0 at solid nodes
+hipError_t macros_extract(const double *macros, const uint8_t *flags, double *out, int k, const Geo &g, hipStream_t s);
 
 // ---- reference mode (FP64, ghost-padded SoA) ----
 struct RefGeo {
@@ -142,9 +136,8 @@ struct lbm_ctx {
     int *block_slot = nullptr;
     double *partials = nullptr;
     int nslots = 0;
-    uint8_t *segcls = nullptr;      // per (row, wave): WAVE_* class of the fused kernel
-    int nseg = 0;
-    uint32_t *links = nullptr;      // per node: LINK_* bits, when some wave is WAVE_NEAR
+    uint32_t *fix = nullptr;        // FP32/16-bit: nodes recomputed per node after the packed path
+    int nfix = 0;
     int V = 1;                      // nodes per lane of the fused kernel
 
     // force history (device + host copy)

@@ -89,8 +89,8 @@ extern "C" int lbm_config_default(lbm_config *c, int lattice) {
 extern "C" const char *lbm_last_error(const lbm_ctx *ctx) { return ctx ? ctx->err.c_str() : t_create_err.c_str(); }
 
 static void free_all(lbm_ctx *c) {
-    void *ptrs[] = {c->A, c->B, c->flags, c->dsfrac, c->duin, c->macros, c->block_slot, c->partials, c->hist, c->segcls,
-                    c->links, c->rf, c->rft, c->rmac, c->rprof, c->rusr, c->rfpre};
+    void *ptrs[] = {c->A, c->B, c->flags, c->dsfrac, c->duin, c->macros, c->block_slot, c->partials, c->hist, c->fix,
+                    c->rf, c->rft, c->rmac, c->rprof, c->rusr, c->rfpre};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c->A = c->B = nullptr;
@@ -98,9 +98,9 @@ static void free_all(lbm_ctx *c) {
     c->dsfrac = nullptr;
     c->duin = c->macros = c->partials = c->hist = nullptr;
     c->block_slot = nullptr;
-    c->segcls = nullptr;
-    c->links = nullptr;
-    c->rf =c->rft = c->rmac = c->rprof = c->rfpre = nullptr;
+    c->fix = nullptr;
+    c->nfix = 0;
+    c->rf = c->rft = c->rmac = c->rprof = c->rfpre = nullptr;
     c->rusr = nullptr;
 }
 
@@ -364,77 +364,55 @@ static int init_plain(lbm_ctx *c) {
                 flags[node] = near ? FL_NEAR : FL_FLUID;
                 c->near_nodes += near;
             }
-    // force partial slots: blocks holding a node next to the mask or with a solid fraction
     const int V = c->V;
     const Shape sh = lane_shape(g, V);
-    const size_t nblocks = (size_t)sh.gx * g.ny * g.nz;
-    std::vector<int> slot(nblocks, -1);
-    c->nslots = 0;
     const bool sf = !c->sfrac.empty();
-    auto node_class = [&](int x, int y, int z) -> int {   // the wave class a node needs
-        if (x >= g.nx) return WAVE_NODE;
-        if (g.xper && (x == 0 || x == g.nx - 1)) return WAVE_NODE;
+    auto forced = [&](int x, int y, int z) {   // a node whose links or solid fraction carry force
+        if (x >= g.nx) return false;
         const size_t node = (size_t)x + (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
-        if (sf && flags[node] != FL_SOLID && c->sfrac[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] > 0.0)
-            return WAVE_NODE;
-        return flags[node] == FL_FLUID ? WAVE_FAST : WAVE_NEAR;
+        return flags[node] == FL_NEAR ||
+               (sf && flags[node] != FL_SOLID && c->sfrac[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] > 0.0);
     };
-    for (int z = 0; z < g.nz; z++)
-        for (int y = 0; y < g.ny; y++)
-            for (int bxi = 0; bxi < sh.gx; bxi++) {
-                bool any = false;
-                for (int x = bxi * sh.bx * V; x < std::min(g.nx, (bxi + 1) * sh.bx * V) && !any; x++) {
-                    const size_t node = (size_t)x + (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
-                    any = flags[node] == FL_NEAR ||
-                          (sf && flags[node] != FL_SOLID && c->sfrac[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] > 0.0);
+    // FP64 per-node kernel: a force partial slot per block holding a forced node
+    std::vector<int> slot;
+    c->nslots = 0;
+    if (c->prec == LBM_FP64) {
+        slot.assign((size_t)sh.gx * g.ny * g.nz, -1);
+        for (int z = 0; z < g.nz; z++)
+            for (int y = 0; y < g.ny; y++)
+                for (int bxi = 0; bxi < sh.gx; bxi++) {
+                    bool any = false;
+                    for (int x = bxi * sh.bx * V; x < std::min(g.nx, (bxi + 1) * sh.bx * V) && !any; x++)
+                        any = forced(x, y, z);
+                    if (any) slot[(size_t)bxi + (size_t)sh.gx * ((size_t)y + (size_t)g.ny * z)] = c->nslots++;
                 }
-                if (any) slot[(size_t)bxi + (size_t)sh.gx * ((size_t)y + (size_t)g.ny * z)] = c->nslots++;
-            }
-    // per (row, 64-lane wave): the most general class any of its nodes needs (fused FP32/16-bit
-    // kernel); LBM_FORCE_SLOW_PATH=1 sends every wave through the per-node path
-    c->nseg = (sh.gx * sh.bx + 63) / 64;
-    const char *force_slow = std::getenv("LBM_FORCE_SLOW_PATH");
-    const bool all_slow = force_slow && force_slow[0] == '1';
-    std::vector<uint8_t> segcls((size_t)c->nseg * g.ny * g.nz, WAVE_FAST);
-    bool any_near = false;
-    for (int z = 0; z < g.nz; z++)
-        for (int y = 0; y < g.ny; y++)
-            for (int s = 0; s < c->nseg; s++) {
-                int cls = all_slow ? WAVE_NODE : WAVE_FAST;
-                for (int x = s * 64 * V; x < (s + 1) * 64 * V && cls != WAVE_NODE; x++)
-                    cls = std::max(cls, node_class(x, y, z));
-                segcls[(size_t)(y + (size_t)g.ny * z) * c->nseg + s] = (uint8_t)cls;
-                any_near |= cls == WAVE_NEAR;
-            }
-    HIP_TRY(c, hipMalloc(&c->segcls, segcls.size()));
-    HIP_TRY(c, hipMemcpyAsync(c->segcls, segcls.data(), segcls.size(), hipMemcpyHostToDevice, c->stream));
-    if (any_near) {
-        // link words: the same source classification as the per-node pull (Zou-He unknowns and
-        // wall links first, outlet clamp, periodic wrap), bit i set when the source is solid
-        std::vector<uint32_t> links(g.plane, 0);
+    }
+    // FP32/16-bit: the nodes the packed path gets wrong, recomputed per node after it (row-major
+    // padded indices): neighbours of solid nodes, nodes with a solid fraction, the periodic x
+    // edges, and the last node when nx is not a multiple of the lane width V (its lane reaches
+    // into the padding).  LBM_FORCE_SLOW_PATH=1 lists every fluid node.  A force partial slot
+    // per forced node, numbered in node order, so the force sum does not depend on the list.
+    if (c->prec != LBM_FP64) {
+        const char *force_slow = std::getenv("LBM_FORCE_SLOW_PATH");
+        const bool all_slow = force_slow && force_slow[0] == '1';
+        std::vector<uint32_t> fix;
         for (int z = 0; z < g.nz; z++)
             for (int y = 0; y < g.ny; y++)
                 for (int x = 0; x < g.nx; x++) {
                     const size_t node = (size_t)x + (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
-                    if (flags[node] == FL_SOLID) {
-                        links[node] = LINK_SOLID;
-                        continue;
-                    }
-                    if (flags[node] != FL_NEAR) continue;
-                    uint32_t bits = 0;
-                    for (int i = 1; i < Q; i++) {
-                        const int ex = e[3 * i], ey = e[3 * i + 1], ez = e[3 * i + 2];
-                        if (ex > 0 && x == 0 && !g.xper) continue;
-                        int sx = x - ex, sy = y - ey, sz = z - ez;
-                        if ((sy < 0 || sy >= g.ny) && !g.yper) continue;
-                        if ((sz < 0 || sz >= g.nz) && !g.zper) continue;
-                        if (sx == g.nx && !g.xper) sx = g.nx - 1;
-                        if (solid_at(sx, sy, sz)) bits |= 1u << i;
-                    }
-                    links[node] = bits;
+                    if (flags[node] == FL_SOLID) continue;
+                    const bool f = forced(x, y, z);
+                    const bool edge = (g.xper && (x == 0 || x == g.nx - 1)) || (x == g.nx - 1 && g.nx % V != 0);
+                    if (!(all_slow || f || edge)) continue;
+                    fix.push_back((uint32_t)node);
+                    slot.push_back(f ? c->nslots++ : -1);
                 }
-        HIP_TRY(c, hipMalloc(&c->links, sizeof(uint32_t) * g.plane));
-        HIP_TRY(c, hipMemcpy(c->links, links.data(), sizeof(uint32_t) * g.plane, hipMemcpyHostToDevice));
+        if (fix.size() > (size_t)INT32_MAX) return set_err(c, LBM_ERR_INVALID, "too many boundary nodes");
+        c->nfix = (int)fix.size();
+        if (c->nfix) {
+            HIP_TRY(c, hipMalloc(&c->fix, sizeof(uint32_t) * fix.size()));
+            HIP_TRY(c, hipMemcpy(c->fix, fix.data(), sizeof(uint32_t) * fix.size(), hipMemcpyHostToDevice));
+        }
     }
     const size_t esz = c->prec == LBM_FP64 ? 8 : c->prec == LBM_FP32 ? 4 : 2;
     HIP_TRY(c, hipMalloc(&c->A, esz * Q * g.plane));
@@ -442,11 +420,12 @@ static int init_plain(lbm_ctx *c) {
     HIP_TRY(c, hipMalloc(&c->flags, g.plane));
     HIP_TRY(c, hipMalloc(&c->duin, sizeof(double) * c->uin.size()));
     HIP_TRY(c, hipMalloc(&c->macros, sizeof(double) * 4 * g.plane));
-    HIP_TRY(c, hipMalloc(&c->block_slot, sizeof(int) * nblocks));
-    HIP_TRY(c, hipMalloc(&c->partials, sizeof(double) * 3 * std::max(c->nslots, 1)));
+    HIP_TRY(c, hipMalloc(&c->block_slot, sizeof(int) * std::max<size_t>(slot.size(), 1)));
+    HIP_TRY(c, hipMalloc(&c->partials, sizeof(double) * 3 * (std::max(c->nslots, 1) + force_chunks(c->nslots))));
     HIP_TRY(c, hipMemcpyAsync(c->flags, flags.data(), g.plane, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->duin, c->uin.data(), sizeof(double) * c->uin.size(), hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(c->block_slot, slot.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, c->stream));
+    if (!slot.empty())
+        HIP_TRY(c, hipMemcpy(c->block_slot, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice));
     if (sf) {
         const size_t ssz = c->prec == LBM_FP64 ? 8 : 4;
         std::vector<char> buf(ssz * g.plane, 0);
@@ -545,9 +524,8 @@ StepBufs step_bufs(const lbm_ctx *c, const void *A, void *B) {
     b.A = A;
     b.B = B;
     b.flags = c->flags;
-    b.segcls = c->segcls;
-    b.nseg = c->nseg;
-    b.links = c->links;
+    b.fix = c->fix;
+    b.nfix = c->nfix;
     b.sfrac = c->dsfrac;
     b.uin = c->duin;
     b.block_slot = c->block_slot;
@@ -580,7 +558,7 @@ static int step_n(lbm_ctx *c, int n) {
         const bool force = fi > 0 && c->step % fi == 0;
         const StepBufs b = step_bufs(c, c->A, c->B);
         HIP_TRY(c, plain_step(c->Q, c->prec, c->V, b, force, k == n - 1, a, c->stream));
-        if (force) HIP_TRY(c, force_reduce(c->partials, c->nslots, c->hist, c->nrec++, c->step, c->stream));
+        if (force) HIP_TRY(c, force_reduce(c->partials, c->nslots, c->partials + 3 * (size_t)c->nslots, c->hist, c->nrec++, c->step, c->stream));
         std::swap(c->A, c->B);
     }
     return LBM_OK;
@@ -662,7 +640,7 @@ extern "C" int lbm_get_fields(lbm_ctx *c, double *p, double *u, double *v, doubl
     HIP_TRY(c, hipMalloc(&tmp, sizeof(double) * c->N));
     for (int k = 0; k < 4; k++) {
         if (!outs[k]) continue;
-        hipError_t e = macros_extract(c->macros, tmp, k, c->geo, c->stream);
+        hipError_t e = macros_extract(c->macros, c->flags, tmp, k, c->geo, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(outs[k], tmp, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) {

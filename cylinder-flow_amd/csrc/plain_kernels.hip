@@ -4,7 +4,7 @@
 // -> stream1 -> stream2 -> Update sequence (3D MPI+KOKKOS+MPIIO/lbm.cpp:257-417;
 // MPI Kokkos/lbm.cpp:127-139,313-457).  Each lane owns one node and
 //   1. pulls Q populations from buffer A (neighbour x - e_i, or a boundary rule),
-//   2. applies the Zou-He velocity inlet at x = 0,
+//   2. applies the velocity inlet at x = 0 (Zou-He, regularised),
 //   3. takes the moments p = sum g / 3, u = sum g e (the reference's Update),
 //   4. relaxes toward the shifted incompressible equilibrium (the reference's Collision,
 //      for the next step) and applies partial bounce-back where a solid fraction is set,
@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(512)
     // ---- 1. pull ----
     pull<Q, S, FORCE>(A, flags, g, a.sp, x, y, z, node, live, near, gv, fx, fy, fz);
 
-    // ---- 2. Zou-He inlet ----
+    // ---- 2. velocity inlet ----
     if (!g.xper && x == 0 && live) velocity_inlet<Q, C>(gv, (C)uin[y + g.ny * z]);
 
     // ---- 3./4. moments, collision, partial bounce-back, store ----
@@ -179,7 +179,31 @@ __global__ void __launch_bounds__(512)
     }
 }
 
-// second pass of the force reduction: fixed order, one block of 256 lanes
+// Force reduction after the per-block / per-node partials.  Above FORCE_CHUNK slots a first
+// pass sums fixed chunks of FORCE_CHUNK slots in parallel (block b: slots [b*CHUNK, (b+1)*CHUNK)),
+// then one block sums the chunk sums; both passes use a fixed order, so the result is
+// deterministic and depends only on the slot values.
+constexpr int FORCE_CHUNK = 4096;
+
+__global__ void __launch_bounds__(256) k_force_chunks(const double *__restrict__ partials, int nslots,
+                                                      double *__restrict__ out) {
+    const int lo = blockIdx.x * FORCE_CHUNK, hi = min(nslots, lo + FORCE_CHUNK);
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int j = lo + (int)threadIdx.x; j < hi; j += 256) {
+        a += partials[3 * (size_t)j];
+        b += partials[3 * (size_t)j + 1];
+        c += partials[3 * (size_t)j + 2];
+    }
+    __shared__ double red[3 * 4];
+    block_sum3<4>(a, b, c, red);
+    if (threadIdx.x == 0) {
+        out[3 * (size_t)blockIdx.x] = a;
+        out[3 * (size_t)blockIdx.x + 1] = b;
+        out[3 * (size_t)blockIdx.x + 2] = c;
+    }
+}
+
+// final pass: fixed order, one block of 256 lanes
 __global__ void __launch_bounds__(256) k_force_reduce(const double *__restrict__ partials, int nslots,
                                                       double *__restrict__ hist, int rec, long step) {
     double a = 0.0, b = 0.0, c = 0.0;
@@ -218,12 +242,12 @@ __global__ void __launch_bounds__(512) k_encode(typename S::T *__restrict__ A, c
     A[node] = S::encode(flags[node] == FL_SOLID ? 0.0 : in[phys], winv, offw);
 }
 
-__global__ void __launch_bounds__(512) k_macros_extract(const double *__restrict__ mac, double *__restrict__ out,
-                                                        Geo g) {
+__global__ void __launch_bounds__(512) k_macros_extract(const double *__restrict__ mac, const uint8_t *__restrict__ flags,
+                                                        double *__restrict__ out, Geo g) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
     if (x >= g.nx) return;
     const size_t node = (size_t)x + (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
-    out[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] = mac[node];
+    out[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] = flags[node] == FL_SOLID ? 0.0 : mac[node];
 }
 
 // ------------------------------------------------------------------------------------
@@ -304,7 +328,18 @@ hipError_t plain_step(int Q, int prec, int V, const StepBufs &b, bool force, boo
     }
 }
 
-hipError_t force_reduce(const double *partials, int nslots, double *hist, int rec, long step, hipStream_t s) {
+int force_chunks(int nslots) { return nslots > FORCE_CHUNK ? (nslots + FORCE_CHUNK - 1) / FORCE_CHUNK : 0; }
+
+hipError_t force_reduce(const double *partials, int nslots, double *chunks, double *hist, int rec, long step,
+                        hipStream_t s) {
+    const int nc = force_chunks(nslots);
+    if (nc > 0) {
+        hipLaunchKernelGGL(k_force_chunks, dim3(nc), dim3(256), 0, s, partials, nslots, chunks);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        partials = chunks;
+        nslots = nc;
+    }
     hipLaunchKernelGGL(k_force_reduce, dim3(1), dim3(256), 0, s, partials, nslots, hist, rec, step);
     return hipGetLastError();
 }
@@ -344,9 +379,9 @@ hipError_t plain_encode(int Q, int prec, void *A, const double *in, const uint8_
     });
 }
 
-hipError_t macros_extract(const double *macros, double *out, int k, const Geo &g, hipStream_t s) {
+hipError_t macros_extract(const double *macros, const uint8_t *flags, double *out, int k, const Geo &g, hipStream_t s) {
     const Shape sh = node_shape(g);
-    hipLaunchKernelGGL(k_macros_extract, grid_of(g, sh), dim3(sh.bx), 0, s, macros + (size_t)k * g.plane, out, g);
+    hipLaunchKernelGGL(k_macros_extract, grid_of(g, sh), dim3(sh.bx), 0, s, macros + (size_t)k * g.plane, flags, out, g);
     return hipGetLastError();
 }
 

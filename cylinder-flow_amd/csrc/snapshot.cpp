@@ -176,7 +176,7 @@ extern "C" int lbm_snapshot_async(lbm_ctx *c, const char *path, int n) {
     } else {
         e = hipMalloc(&dtmp, sizeof(double) * 4 * plane);
         if (e != hipSuccess) return fail(e, "hipMalloc");
-        for (int k = 0; k < 4 && e == hipSuccess; k++) e = macros_extract(c->macros, dtmp + k * plane, k, c->geo, c->stream);
+        for (int k = 0; k < 4 && e == hipSuccess; k++) e = macros_extract(c->macros, c->flags, dtmp + k * plane, k, c->geo, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(pinned, dtmp, sizeof(double) * 4 * plane, hipMemcpyDeviceToHost, c->stream);
         if (e != hipSuccess) return fail(e, "snapshot copy");
     }

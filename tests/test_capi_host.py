@@ -131,3 +131,22 @@ def test_strouhal_of_a_sine(pkg):
     cl = 0.3 * np.sin(2 * np.pi * t / 250.0 + 0.4)
     # period 250 steps sampled every step: St = D / (T u) with D = 20, u = 0.05
     assert abs(pkg.configs.strouhal(cl, 1.0, 20.0, 0.05) - 20.0 / (250.0 * 0.05)) < 1e-3
+    # sampled every 10 steps, with 5 % noise: still the same frequency
+    rng = np.random.default_rng(1)
+    t10 = np.arange(0, 40000, 10)
+    cl10 = np.sin(2 * np.pi * t10 / 2500.0) + 0.05 * rng.standard_normal(t10.size)
+    assert abs(pkg.configs.strouhal(cl10, 10.0, 20.0, 0.05) / (20.0 / (2500.0 * 0.05)) - 1) < 0.02
+
+
+def test_strouhal_band_removes_a_faster_mode(pkg):
+    # shedding at St 0.22 under a stronger mode at St 0.9 (a channel's acoustic standing wave)
+    t = np.arange(0, 60000, 10)
+    D, u = 40.0, 0.05
+    cl = 0.2 * np.sin(2 * np.pi * t * 0.22 * u / D) + 0.3 * np.sin(2 * np.pi * t * 0.9 * u / D)
+    assert abs(pkg.configs.strouhal(cl, 10.0, D, u, band=(0.05, 0.5)) / 0.22 - 1) < 0.02
+
+
+def test_strouhal_is_nan_without_shedding(pkg):
+    rng = np.random.default_rng(2)
+    assert np.isnan(pkg.configs.strouhal(1e-3 * rng.standard_normal(2000), 10.0, 40.0, 0.05))
+    assert np.isnan(pkg.configs.strouhal(np.sin(np.arange(100) / 30.0), 10.0, 40.0, 0.05))   # < 3 periods

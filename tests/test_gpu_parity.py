@@ -190,10 +190,11 @@ def test_adjoint_matches_cpu_adjoint(pkg, orc, Q, shape, prec, tol):
     (9, (1, 16, 128), dict(body=3, mask="edges")),
 ])
 def test_fast_path_matches_per_node_path(pkg, orc, monkeypatch, prec, V, Q, shape, kw):
-    """The wave-level packed paths (V nodes per lane, with and without mask links) and the
-    per-node path give bit-identical populations, fields and forces.  The "edges" mask puts
-    solid nodes on the inlet and outlet lanes, against the walls and as isolated points, so
-    lanes hold both solid and fluid nodes."""
+    """The packed path (V nodes per lane) followed by the per-node recomputation of the nodes
+    it gets wrong, and the per-node path on every fluid node, give bit-identical populations,
+    fields and forces.  The "edges" mask puts solid nodes on the inlet and outlet lanes,
+    against the walls and as isolated points, so lanes hold both solid and fluid nodes; the
+    shapes include rows whose last lane reaches into the padding and periodic x edges."""
     L = pkg.lbm
     nz, ny, nx = shape
     kw = dict(kw)
