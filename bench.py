@@ -131,8 +131,13 @@ def accuracy(pkg, steps, early, window_start=None):
         s.step(early)
         _, u, v, w = s.fields()
         t0 = time.perf_counter()
-        s.step(steps - early)
-        s.sync()
+        done = early
+        while done < steps:
+            n = min(10000, steps - done)
+            s.step(n)
+            s.sync()
+            done += n
+            log(f"accuracy {name}: step {done}")
         dt = time.perf_counter() - t0
         F = s.forces()
         s.close()
