@@ -1,17 +1,20 @@
 // fast_kernel.hpp -- fused pull step for FP32 / FP16 / BF16 storage, V nodes per lane.
 //
 // Same boundary rules and arithmetic as the per-node path (fused_common.hpp), organised for
-// gfx950 memory throughput.  One step is two launches on the solver stream:
+// gfx950 memory throughput.  One step is three launches on the solver stream:
 //  * k_fast_v runs the packed path on every lane of every row: each lane owns V consecutive x
 //    nodes, so each of the 27 pulls and 27 stores is one V*sizeof(storage)-byte access (a
 //    dword for two FP16 nodes); the source row of a direction is a scalar and only three lane
 //    offsets exist (x-1, x, x+1).  It applies walls, periodic y/z, the inlet and the outlet,
-//    but knows nothing of the mask, so nodes next to the body get provisional values and
-//    solid nodes get values that no fluid node ever reads;
-//  * k_fix_v then recomputes, with the per-node pull, every node the packed path gets wrong
-//    (a compact list built once from the mask: neighbours of solid nodes, nodes with a solid
-//    fraction, the periodic x edges, the last node of a lane that reaches into the padding)
-//    and writes the drag/lift partials of the nodes that carry force.
+//    but knows nothing of the mask: solid nodes get values that no fluid node ever reads;
+//  * before it, k_link_v copies, for every mask link of a fluid node x (direction i pulls from
+//    the solid node s = x - e_i), the stored h_opp(i)(x) into slot (s, i) of the input buffer:
+//    the packed path's plain pull then performs the half-way bounce-back itself, and the
+//    copied values give the drag/lift;
+//  * after it, k_fix_v recomputes, with the per-node pull, the few nodes the packed path still
+//    gets wrong (a compact list built once from the mask: nodes with a solid fraction, the
+//    periodic x edges, the last node of a lane that reaches into the padding, mask neighbours
+//    on the outlet column or when the stored offsets differ between opposite directions).
 // The rare boundary work thus costs the fast kernel no registers: it stays at the occupancy
 // of the packed path alone.  Both kernels use collide(), so a node's result does not depend
 // on the path it took.
@@ -246,16 +249,70 @@ __global__ void __launch_bounds__(256)
     packed_nodes<Q, S, V, MACROS>(A, (char *)(B + rowoff), uin, macros, a, tab, y, z, t * V, rowoff);
 }
 
+// Pre-applied mask bounce-back, before k_fast_v on the same stream.  Half-way bounce-back
+// gives a fluid node x pulling direction i from a solid node s = x - e_i the value
+// h_opp(i)(x); slot (s, i) of A is pulled by x alone, so copying h_opp(i)(x) into it makes the
+// packed path's plain pull exact for x.  One lane per listed node (with its link word: bit i
+// set when direction i pulls from a solid node); the copy is of the stored value, exact when
+// opposite directions share their storage offset.  FORCE writes the node's momentum exchange
+// 2 h_opp(i) e_opp(i), summed in direction order as the per-node pull does, to its slot.
+template <int Q, class S, bool FORCE>
+__global__ void __launch_bounds__(256)
+    k_link_v(typename S::T *__restrict__ A, const uint32_t *__restrict__ list, const uint32_t *__restrict__ links,
+             int nlist, const int *__restrict__ slot, double *__restrict__ partials, const FusedArgs a) {
+    using L = Lat<Q>;
+    using T = typename S::T;
+    const Geo &g = a.g;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nlist) return;
+    const uint32_t node = list[j], lk = links[j];
+    const uint32_t row = node / (uint32_t)g.nxp;
+    const int x = (int)(node - row * (uint32_t)g.nxp);
+    const int y = (int)(row % (uint32_t)g.ny), z = (int)(row / (uint32_t)g.ny);
+    T val[Q];
+    static_for<1, Q>([&](auto I) {   // every bounced value first, so the loads are in flight together
+        constexpr int i = I;
+        if ((lk >> i) & 1u) val[i] = A[(size_t)L::opp(i) * g.plane + node];
+    });
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    static_for<1, Q>([&](auto I) {
+        constexpr int i = I;
+        constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i), o = L::opp(i);
+        if ((lk >> i) & 1u) {
+            // the solid source: x - e_i (the list holds no node on a periodic x edge or on the
+            // outlet column, and link bits exclude wall links), periodic wrap in y and z
+            int sy = y - ey, sz = z - ez;
+            if constexpr (ey != 0) sy = sy < 0 ? sy + g.ny : (sy >= g.ny ? sy - g.ny : sy);
+            if constexpr (ez != 0) sz = sz < 0 ? sz + g.nz : (sz >= g.nz ? sz - g.nz : sz);
+            const uint32_t src = (uint32_t)(x - ex) + (uint32_t)g.nxp * ((uint32_t)sy + (uint32_t)g.ny * (uint32_t)sz);
+            A[(size_t)i * g.plane + src] = val[i];
+            if constexpr (FORCE) {
+                const double m = 2.0 * (double)S::template decode<i, L>(val[i], a.sp.off[o]);
+                if constexpr (ex != 0) fx -= ex * m;
+                if constexpr (ey != 0) fy -= ey * m;
+                if constexpr (ez != 0) fz -= ez * m;
+            }
+        }
+    });
+    if constexpr (FORCE) {
+        const int k = slot[j];
+        partials[3 * (size_t)k] = fx;
+        partials[3 * (size_t)k + 1] = fy;
+        partials[3 * (size_t)k + 2] = fz;
+    }
+}
+
 // The per-node recomputation of the listed nodes (padded-layout indices, row-major order),
 // after k_fast_v on the same stream: the general pull of fused_common.hpp (walls, inlet,
-// outlet, periodic axes, mask bounce-back), the velocity inlet, collision with partial
+// outlet, periodic axes, mask bounce-back taken from the node's link word: bit i set when
+// direction i pulls from a solid node), the velocity inlet, collision with partial
 // bounce-back, single-value stores.  FORCE writes each node's momentum exchange (and partial
 // bounce-back force) to partials[3 * slot[j]] when slot[j] >= 0; the slots are numbered in
 // node order, so the later fixed-order sum does not depend on which nodes the list holds.
 template <int Q, class S, bool FORCE, bool MACROS, bool SFRAC>
 __global__ void __launch_bounds__(256)
-    k_fix_v(const typename S::T *__restrict__ A, typename S::T *__restrict__ B, const uint8_t *__restrict__ flags,
-            const uint32_t *__restrict__ list, int nlist, const int *__restrict__ slot,
+    k_fix_v(const typename S::T *__restrict__ A, typename S::T *__restrict__ B, const uint32_t *__restrict__ list,
+            const uint32_t *__restrict__ links, int nlist, const int *__restrict__ slot,
             const float *__restrict__ sfrac, const double *__restrict__ uin, double *__restrict__ partials,
             double *__restrict__ macros, const FusedArgs a, const PlaneTab tab) {
     using L = Lat<Q>;
@@ -267,10 +324,9 @@ __global__ void __launch_bounds__(256)
     const uint32_t row = node / (uint32_t)g.nxp;
     const int x = (int)(node - row * (uint32_t)g.nxp);
     const int y = (int)(row % (uint32_t)g.ny), z = (int)(row / (uint32_t)g.ny);
-    const uint8_t fl = flags[node];
     double fx = 0.0, fy = 0.0, fz = 0.0;
     float gv[Q], c[Q], s, u, v, w;
-    pull<Q, S, FORCE>(A, flags, g, a.sp, x, y, z, node, true, fl == FL_NEAR, gv, fx, fy, fz);
+    pull<Q, S, FORCE, true>(A, nullptr, g, a.sp, x, y, z, node, true, false, gv, fx, fy, fz, links[j]);
     if (!g.xper && x == 0) velocity_inlet<Q, float>(gv, (float)uin[y + g.ny * z]);
     float sf = 0.f;
     if constexpr (SFRAC) sf = sfrac[node];
@@ -297,23 +353,29 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// One step: the packed path everywhere, then the listed nodes.
+// One step, three launches on s: the bounce-back copies into A's solid slots, the packed path
+// everywhere, then the listed nodes.
 template <int Q, class S, int V, bool FORCE, bool MACROS>
 static hipError_t fast_launch(const StepBufs &b, const FusedArgs &a, hipStream_t s) {
     using T = typename S::T;
     const Shape sh = lane_shape(a.g, V);
     const PlaneTab tab = plane_tab<Q, T>(a.g);
+    hipError_t e;
+    if (b.nlink > 0) {
+        hipLaunchKernelGGL((k_link_v<Q, S, FORCE>), dim3((unsigned)((b.nlink + 255) / 256)), dim3(256), 0, s,
+                           (T *)b.A, b.link, b.linklk, b.nlink, b.link_slot, b.partials, a);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     hipLaunchKernelGGL((k_fast_v<Q, S, V, MACROS>), dim3(sh.gx, a.g.ny, a.g.nz), dim3(sh.bx), 0, s, (const T *)b.A,
                        (T *)b.B, b.uin, b.macros, a, tab);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || b.nfix == 0) return e;
+    if ((e = hipGetLastError()) != hipSuccess || b.nfix == 0) return e;
     const dim3 grid((unsigned)((b.nfix + 255) / 256));
     if (b.sfrac)
-        hipLaunchKernelGGL((k_fix_v<Q, S, FORCE, MACROS, true>), grid, dim3(256), 0, s, (const T *)b.A, (T *)b.B, b.flags,
-                           b.fix, b.nfix, b.block_slot, (const float *)b.sfrac, b.uin, b.partials, b.macros, a, tab);
+        hipLaunchKernelGGL((k_fix_v<Q, S, FORCE, MACROS, true>), grid, dim3(256), 0, s, (const T *)b.A, (T *)b.B, b.fix,
+                           b.fixlk, b.nfix, b.block_slot, (const float *)b.sfrac, b.uin, b.partials, b.macros, a, tab);
     else
-        hipLaunchKernelGGL((k_fix_v<Q, S, FORCE, MACROS, false>), grid, dim3(256), 0, s, (const T *)b.A, (T *)b.B, b.flags,
-                           b.fix, b.nfix, b.block_slot, (const float *)nullptr, b.uin, b.partials, b.macros, a, tab);
+        hipLaunchKernelGGL((k_fix_v<Q, S, FORCE, MACROS, false>), grid, dim3(256), 0, s, (const T *)b.A, (T *)b.B, b.fix,
+                           b.fixlk, b.nfix, b.block_slot, (const float *)nullptr, b.uin, b.partials, b.macros, a, tab);
     return hipGetLastError();
 }
 

@@ -159,13 +159,17 @@ __device__ __forceinline__ void block_sum3(double &a, double &b, double &c, doub
 //   source is solid (mask)                         -> half-way bounce-back, momentum exchange
 //   otherwise                                      -> g_i = h_i(node - e_i) (periodic wrap)
 // FORCE accumulates the momentum exchange 2 h_opp(i) e_opp(i) of the mask links.
+// Which sources are solid: LINKS = false gathers the flags of the Q sources when `near`;
+// LINKS = true takes them from `lk` (bit i: direction i pulls from a solid node), computed
+// once on the host with the same rules, so no flag is read.
 // `node` must be a valid element of the plane even when the lane is not live; every load
 // below is issued unconditionally from a valid address (no data-dependent branches), so the
 // Q loads of a lane are in flight together.
-template <int Q, class S, bool FORCE>
+template <int Q, class S, bool FORCE, bool LINKS = false>
 __device__ __forceinline__ void pull(const typename S::T *__restrict__ A, const uint8_t *__restrict__ flags,
                                      const Geo &g, const StoreParams &sp, int x, int y, int z, uint32_t node,
-                                     bool live, bool near, typename S::C *gv, double &fx, double &fy, double &fz) {
+                                     bool live, bool near, typename S::C *gv, double &fx, double &fy, double &fz,
+                                     uint32_t lk = 0) {
     using L = Lat<Q>;
     using T = typename S::T;
     using C = typename S::C;
@@ -203,7 +207,9 @@ __device__ __forceinline__ void pull(const typename S::T *__restrict__ A, const 
         src[i] = plain ? (uint32_t)sx + (uint32_t)g.nxp * ((uint32_t)sy + (uint32_t)g.ny * (uint32_t)sz) : node;
         solid[i] = false;
     });
-    if (near) {   // only nodes next to the body look at their sources' flags
+    if constexpr (LINKS) {
+        static_for<0, Q>([&](auto I) { solid[I] = !zh[I] && !wall[I] && ((lk >> (int)I) & 1u); });
+    } else if (near) {   // only nodes next to the body look at their sources' flags
         uint8_t fl[Q];
         static_for<0, Q>([&](auto I) { fl[I] = flags[src[I]]; });
         static_for<0, Q>([&](auto I) { solid[I] = !zh[I] && !wall[I] && fl[I] == FL_SOLID; });

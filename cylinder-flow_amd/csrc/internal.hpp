@@ -49,7 +49,12 @@ struct StepBufs {
     void *B;                    // populations out
     const uint8_t *flags;       // FL_* per node (padded layout)
     const uint32_t *fix;        // FP32/16-bit: nodes the packed path gets wrong (padded indices)
+    const uint32_t *fixlk;      // their link words: bit i set when direction i pulls from a solid node
     int nfix;
+    const uint32_t *link;       // FP32/16-bit: fluid nodes whose mask links are pre-applied by copies
+    const uint32_t *linklk;     // their link words
+    const int *link_slot;       // their force partial slots
+    int nlink;
     const void *sfrac;          // solid fraction (FP64: double, else float) or null
     const double *uin;          // inlet velocity per (y, z)
     const int *block_slot;      // force partial slot per block (FP64) or per listed node (-1: none)
@@ -137,7 +142,12 @@ struct lbm_ctx {
     double *partials = nullptr;
     int nslots = 0;
     uint32_t *fix = nullptr;        // FP32/16-bit: nodes recomputed per node after the packed path
+    uint32_t *fixlk = nullptr;      // their link words
     int nfix = 0;
+    uint32_t *link = nullptr;       // FP32/16-bit: nodes whose mask links are copied before the packed path
+    uint32_t *linklk = nullptr;     // their link words
+    int *link_slot = nullptr;       // their force slots
+    int nlink = 0;
     int V = 1;                      // nodes per lane of the fused kernel
 
     // force history (device + host copy)

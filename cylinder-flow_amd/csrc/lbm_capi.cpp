@@ -90,7 +90,7 @@ extern "C" const char *lbm_last_error(const lbm_ctx *ctx) { return ctx ? ctx->er
 
 static void free_all(lbm_ctx *c) {
     void *ptrs[] = {c->A, c->B, c->flags, c->dsfrac, c->duin, c->macros, c->block_slot, c->partials, c->hist, c->fix,
-                    c->rf, c->rft, c->rmac, c->rprof, c->rusr, c->rfpre};
+                    c->fixlk, c->link, c->linklk, c->link_slot, c->rf, c->rft, c->rmac, c->rprof, c->rusr, c->rfpre};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c->A = c->B = nullptr;
@@ -98,8 +98,9 @@ static void free_all(lbm_ctx *c) {
     c->dsfrac = nullptr;
     c->duin = c->macros = c->partials = c->hist = nullptr;
     c->block_slot = nullptr;
-    c->fix = nullptr;
-    c->nfix = 0;
+    c->fix = c->fixlk = c->link = c->linklk = nullptr;
+    c->link_slot = nullptr;
+    c->nfix = c->nlink = 0;
     c->rf = c->rft = c->rmac = c->rprof = c->rfpre = nullptr;
     c->rusr = nullptr;
 }
@@ -129,7 +130,12 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_ctx **out) {
     if (cfg->ncoef < 0 || cfg->ncoef > 8) return set_err(nullptr, LBM_ERR_INVALID, "ncoef must be in [0, 8]");
     if (cfg->force_interval < 0) return set_err(nullptr, LBM_ERR_INVALID, "force_interval must be >= 0");
     const int nxp = (cfg->nx + 63) / 64 * 64;
-    const size_t plane = (size_t)nxp * cfg->ny * nz;
+    // direction planes are `plane` elements apart: the nodes of one plane plus an optional gap
+    // (LBM_PLANE_PAD elements, rounded up to 64), so that the Q planes need not sit a power of
+    // two apart in memory
+    size_t pad = 0;
+    if (const char *env = std::getenv("LBM_PLANE_PAD")) pad = ((size_t)std::max(0L, std::atol(env)) + 63) / 64 * 64;
+    const size_t plane = (size_t)nxp * cfg->ny * nz + pad;
     if (plane >= (size_t)1 << 32) return set_err(nullptr, LBM_ERR_INVALID, "domain too large: a direction plane must stay below 2^32 nodes");
 
     lbm_ctx *c = new lbm_ctx();
@@ -387,31 +393,75 @@ static int init_plain(lbm_ctx *c) {
                     if (any) slot[(size_t)bxi + (size_t)sh.gx * ((size_t)y + (size_t)g.ny * z)] = c->nslots++;
                 }
     }
-    // FP32/16-bit: the nodes the packed path gets wrong, recomputed per node after it (row-major
-    // padded indices): neighbours of solid nodes, nodes with a solid fraction, the periodic x
-    // edges, and the last node when nx is not a multiple of the lane width V (its lane reaches
-    // into the padding).  LBM_FORCE_SLOW_PATH=1 lists every fluid node.  A force partial slot
-    // per forced node, numbered in node order, so the force sum does not depend on the list.
+    // FP32/16-bit: per fluid node (row-major padded indices), either
+    //  * the link list: mask neighbours whose bounce-back is pre-applied by copying h_opp(i)
+    //    into the solid source's slot i before the packed path (exact when opposite
+    //    directions share their storage offset: FP32, or 16-bit storage with store_uref 0), or
+    //  * the fix list, recomputed per node after the packed path: nodes with a solid fraction,
+    //    the periodic x edges, the last node when nx is not a multiple of the lane width V
+    //    (its lane reaches into the padding), and mask neighbours on the outlet column (whose
+    //    clamped pull shares a solid slot with its x-1 neighbour) or when copies are not exact.
+    // LBM_FORCE_SLOW_PATH=1 puts every fluid node on the fix list.  Force partial slots go to
+    // the forced nodes in node order across both lists, so the force sum does not depend on
+    // which list a node is on.
     if (c->prec != LBM_FP64) {
         const char *force_slow = std::getenv("LBM_FORCE_SLOW_PATH");
         const bool all_slow = force_slow && force_slow[0] == '1';
-        std::vector<uint32_t> fix;
+        const bool copies_exact = c->prec == LBM_FP32 || c->cfg.store_uref == 0.0;
+        std::vector<uint32_t> fix, fixlk, link, linklk;
+        std::vector<int> lslot;
         for (int z = 0; z < g.nz; z++)
             for (int y = 0; y < g.ny; y++)
                 for (int x = 0; x < g.nx; x++) {
                     const size_t node = (size_t)x + (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
                     if (flags[node] == FL_SOLID) continue;
-                    const bool f = forced(x, y, z);
+                    const bool f = forced(x, y, z), near = flags[node] == FL_NEAR;
+                    const bool sfr = sf && c->sfrac[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] > 0.0;
                     const bool edge = (g.xper && (x == 0 || x == g.nx - 1)) || (x == g.nx - 1 && g.nx % V != 0);
-                    if (!(all_slow || f || edge)) continue;
-                    fix.push_back((uint32_t)node);
-                    slot.push_back(f ? c->nslots++ : -1);
+                    const bool to_fix = all_slow || sfr || edge || (near && (x == g.nx - 1 || !copies_exact));
+                    if (!to_fix && !near) continue;
+                    // link word: the per-node pull's classification (Zou-He unknowns and wall
+                    // links first, outlet clamp, periodic wrap), bit i when the source is solid
+                    uint32_t bits = 0;
+                    if (near)
+                        for (int i = 1; i < Q; i++) {
+                            const int ex = e[3 * i], ey = e[3 * i + 1], ez = e[3 * i + 2];
+                            if (ex > 0 && x == 0 && !g.xper) continue;
+                            int sx = x - ex;
+                            const int sy = y - ey, sz = z - ez;
+                            if ((sy < 0 || sy >= g.ny) && !g.yper) continue;
+                            if ((sz < 0 || sz >= g.nz) && !g.zper) continue;
+                            if (sx == g.nx && !g.xper) sx = g.nx - 1;
+                            if (solid_at(sx, sy, sz)) bits |= 1u << i;
+                        }
+                    const int k = f ? c->nslots++ : -1;
+                    if (to_fix) {
+                        fix.push_back((uint32_t)node);
+                        fixlk.push_back(bits);
+                        slot.push_back(k);
+                    } else {
+                        link.push_back((uint32_t)node);
+                        linklk.push_back(bits);
+                        lslot.push_back(k);
+                    }
                 }
-        if (fix.size() > (size_t)INT32_MAX) return set_err(c, LBM_ERR_INVALID, "too many boundary nodes");
+        if (fix.size() > (size_t)INT32_MAX || link.size() > (size_t)INT32_MAX)
+            return set_err(c, LBM_ERR_INVALID, "too many boundary nodes");
         c->nfix = (int)fix.size();
+        c->nlink = (int)link.size();
         if (c->nfix) {
             HIP_TRY(c, hipMalloc(&c->fix, sizeof(uint32_t) * fix.size()));
             HIP_TRY(c, hipMemcpy(c->fix, fix.data(), sizeof(uint32_t) * fix.size(), hipMemcpyHostToDevice));
+            HIP_TRY(c, hipMalloc(&c->fixlk, sizeof(uint32_t) * fixlk.size()));
+            HIP_TRY(c, hipMemcpy(c->fixlk, fixlk.data(), sizeof(uint32_t) * fixlk.size(), hipMemcpyHostToDevice));
+        }
+        if (c->nlink) {
+            HIP_TRY(c, hipMalloc(&c->link, sizeof(uint32_t) * link.size()));
+            HIP_TRY(c, hipMemcpy(c->link, link.data(), sizeof(uint32_t) * link.size(), hipMemcpyHostToDevice));
+            HIP_TRY(c, hipMalloc(&c->linklk, sizeof(uint32_t) * linklk.size()));
+            HIP_TRY(c, hipMemcpy(c->linklk, linklk.data(), sizeof(uint32_t) * linklk.size(), hipMemcpyHostToDevice));
+            HIP_TRY(c, hipMalloc(&c->link_slot, sizeof(int) * lslot.size()));
+            HIP_TRY(c, hipMemcpy(c->link_slot, lslot.data(), sizeof(int) * lslot.size(), hipMemcpyHostToDevice));
         }
     }
     const size_t esz = c->prec == LBM_FP64 ? 8 : c->prec == LBM_FP32 ? 4 : 2;
@@ -525,7 +575,12 @@ StepBufs step_bufs(const lbm_ctx *c, const void *A, void *B) {
     b.B = B;
     b.flags = c->flags;
     b.fix = c->fix;
+    b.fixlk = c->fixlk;
     b.nfix = c->nfix;
+    b.link = c->link;
+    b.linklk = c->linklk;
+    b.link_slot = c->link_slot;
+    b.nlink = c->nlink;
     b.sfrac = c->dsfrac;
     b.uin = c->duin;
     b.block_slot = c->block_slot;
