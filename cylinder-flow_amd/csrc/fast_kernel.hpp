@@ -5,8 +5,11 @@
 //  * k_fast_v runs the packed path on every lane of every row: each lane owns V consecutive x
 //    nodes, so each of the 27 pulls and 27 stores is one V*sizeof(storage)-byte access (a
 //    dword for two FP16 nodes); the source row of a direction is a scalar and only three lane
-//    offsets exist (x-1, x, x+1).  It applies walls, periodic y/z, the inlet and the outlet,
-//    but knows nothing of the mask: solid nodes get values that no fluid node ever reads;
+//    offsets exist (x-1, x, x+1).  For two 16-bit nodes per lane, interior rows load every
+//    direction aligned and build the x-1 / x+1 pairs from the neighbouring lane's dword
+//    (aligned_pull), so a wave's request spans two cache lines instead of three.  It applies
+//    walls, periodic y/z, the inlet and the outlet, but knows nothing of the mask: solid nodes
+//    get values that no fluid node ever reads;
 //  * before it, k_link_v copies, for every mask link of a fluid node x (direction i pulls from
 //    the solid node s = x - e_i), the stored h_opp(i)(x) into slot (s, i) of the input buffer:
 //    the packed path's plain pull then performs the half-way bounce-back itself, and the
@@ -87,6 +90,15 @@ __device__ __forceinline__ void collide(const float *g, float sf, float om, floa
     }
 }
 
+// rank of direction i among the directions with the same sign of e_x
+template <int Q>
+constexpr int xrank(int i) {
+    int k = 0;
+    for (int j = 0; j < i; j++)
+        if ((Lat<Q>::ex(j) > 0) == (Lat<Q>::ex(i) > 0) && (Lat<Q>::ex(j) < 0) == (Lat<Q>::ex(i) < 0)) k++;
+    return k;
+}
+
 // V storage values of one direction, loaded and stored as one access
 template <class T, int V>
 struct Pack {
@@ -137,12 +149,55 @@ inline PlaneTab plane_tab(const Geo &g) {
     return t;
 }
 
+// Aligned pull of an interior row for two 16-bit nodes per lane, the whole row in one block
+// (up to 4 waves).  Every direction is loaded as the lane's own aligned dword D_t = (h[2t],
+// h[2t+1]) of its source row; e_x = +1 needs (h[2t-1], h[2t]), the high half of the previous
+// lane's dword and the low half of its own, e_x = -1 needs (h[2t+1], h[2t+2]).  The
+// neighbour's dword comes from a wave shift (DPP wave_shr:1 / wave_shl:1); at a wave edge the
+// shift returns the `old` operand, which holds the dword that the neighbouring wave's edge
+// lane left in LDS.  The inlet lane (x = 0) and the lane holding the last node keep their own
+// dword, which the caller's clamp then shifts as on the unaligned path.  Each request covers
+// two 128-byte lines instead of three.
+template <int Q, class T>
+__device__ __forceinline__ void aligned_pull(const char *Arow, const PlaneTab &tab, uint32_t b0, bool first,
+                                             bool last, Pack<T, 2> *r) {
+    using L = Lat<Q>;
+    constexpr int NS = Q == 27 ? 9 : 3;   // directions per sign of e_x
+    __shared__ uint32_t up[4][NS], dn[4][NS];
+    const int lane = (int)(threadIdx.x & 63u), wv = (int)(threadIdx.x >> 6);
+    const int nw = (int)((blockDim.x + 63u) >> 6);
+    uint32_t D[Q];
+    static_for<0, Q>([&](auto I) { D[I] = load_pack<uint32_t, 1>(Arow + tab.src[I] + b0).e[0]; });
+    if (lane == 63)
+        static_for<0, Q>([&](auto I) {
+            if constexpr (L::ex(I) > 0) up[wv][xrank<Q>(I)] = D[I];
+        });
+    if (lane == 0)
+        static_for<0, Q>([&](auto I) {
+            if constexpr (L::ex(I) < 0) dn[wv][xrank<Q>(I)] = D[I];
+        });
+    __syncthreads();
+    const int wp = wv > 0 ? wv - 1 : 0, wn = wv + 1 < nw ? wv + 1 : wv;
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = I;
+        uint32_t v = D[i];
+        if constexpr (L::ex(i) > 0) {
+            const int P = __builtin_amdgcn_update_dpp((int)up[wp][xrank<Q>(i)], (int)D[i], 0x138, 0xf, 0xf, false);
+            v = first ? D[i] : __builtin_amdgcn_alignbit(D[i], (uint32_t)P, 16);
+        } else if constexpr (L::ex(i) < 0) {
+            const int N = __builtin_amdgcn_update_dpp((int)dn[wn][xrank<Q>(i)], (int)D[i], 0x130, 0xf, 0xf, false);
+            v = last ? D[i] : __builtin_amdgcn_alignbit((uint32_t)N, D[i], 16);
+        }
+        __builtin_memcpy(&r[i], &v, 4);
+    });
+}
+
 // Packed path of one lane: V consecutive nodes of one x row.  Interior rows (no wall, no
 // periodic wrap in y/z) address direction i at the scalar Arow + tab.src[i]; boundary rows pick
 // the source row (or the wall bounce-back row) per direction.  The inlet lane (x = 0) and the
 // lane holding the outlet node (x = nx-1) load their row aligned and shift one value; a lane
 // past the outlet (padding) clamps the same way, so every access stays inside its row.
-template <int Q, class S, int V, bool MACROS>
+template <int Q, class S, int V, bool MACROS, bool ALIGNED>
 __device__ __forceinline__ void packed_nodes(const typename S::T *__restrict__ A, char *__restrict__ Brow,
                                              const double *__restrict__ uin, double *__restrict__ macros,
                                              const FusedArgs &a, const PlaneTab &tab, int y, int z, int x0,
@@ -159,7 +214,10 @@ __device__ __forceinline__ void packed_nodes(const typename S::T *__restrict__ A
                           (L::D == 2 || (unsigned)(z - 1) < (unsigned)(g.nz - 2));
     Pack<T, V> r[Q];
     bool bbw[Q];   // wall bounce-back: uniform over the block
-    if (interior) {
+    if (ALIGNED && interior) {
+        if constexpr (ALIGNED) aligned_pull<Q, T>(Arow, tab, b0, first, last, r);
+        static_for<0, Q>([&](auto I) { bbw[I] = false; });
+    } else if (interior) {
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
             constexpr int ex = L::ex(i);
@@ -237,7 +295,7 @@ __device__ __forceinline__ void packed_nodes(const typename S::T *__restrict__ A
 // Every lane of every row (256-lane blocks along x, one row per (blockIdx.y, blockIdx.z)):
 // lanes past the padded row return; nothing else branches on the node, and there is no
 // barrier.
-template <int Q, class S, int V, bool MACROS>
+template <int Q, class S, int V, bool MACROS, bool ALIGNED>
 __global__ void __launch_bounds__(256)
     k_fast_v(const typename S::T *__restrict__ A, typename S::T *__restrict__ B, const double *__restrict__ uin,
              double *__restrict__ macros, const FusedArgs a, const PlaneTab tab) {
@@ -246,7 +304,7 @@ __global__ void __launch_bounds__(256)
     if (t * V >= g.nxp) return;
     const int y = blockIdx.y, z = blockIdx.z;
     const size_t rowoff = (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
-    packed_nodes<Q, S, V, MACROS>(A, (char *)(B + rowoff), uin, macros, a, tab, y, z, t * V, rowoff);
+    packed_nodes<Q, S, V, MACROS, ALIGNED>(A, (char *)(B + rowoff), uin, macros, a, tab, y, z, t * V, rowoff);
 }
 
 // Pre-applied mask bounce-back, before k_fast_v on the same stream.  Half-way bounce-back
@@ -366,8 +424,14 @@ static hipError_t fast_launch(const StepBufs &b, const FusedArgs &a, hipStream_t
                            (T *)b.A, b.link, b.linklk, b.nlink, b.link_slot, b.partials, a);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_fast_v<Q, S, V, MACROS>), dim3(sh.gx, a.g.ny, a.g.nz), dim3(sh.bx), 0, s, (const T *)b.A,
-                       (T *)b.B, b.uin, b.macros, a, tab);
+    // the aligned pull needs the whole row in one block
+    const unsigned lds = b.fast_lds;
+    if (sizeof(T) == 2 && V == 2 && sh.gx == 1 && b.aligned_pull)
+        hipLaunchKernelGGL((k_fast_v<Q, S, V, MACROS, (sizeof(T) == 2 && V == 2)>), dim3(sh.gx, a.g.ny, a.g.nz),
+                           dim3(sh.bx), lds, s, (const T *)b.A, (T *)b.B, b.uin, b.macros, a, tab);
+    else
+        hipLaunchKernelGGL((k_fast_v<Q, S, V, MACROS, false>), dim3(sh.gx, a.g.ny, a.g.nz), dim3(sh.bx), lds, s,
+                           (const T *)b.A, (T *)b.B, b.uin, b.macros, a, tab);
     if ((e = hipGetLastError()) != hipSuccess || b.nfix == 0) return e;
     const dim3 grid((unsigned)((b.nfix + 255) / 256));
     if (b.sfrac)

@@ -60,6 +60,8 @@ struct StepBufs {
     const int *block_slot;      // force partial slot per block (FP64) or per listed node (-1: none)
     double *partials;
     double *macros;             // p, u, v, w planes (written when macros_on)
+    bool aligned_pull;          // 16-bit, two nodes per lane: aligned loads + lane shifts (k_fast_v)
+    unsigned fast_lds;          // dynamic LDS bytes per k_fast_v block (caps its blocks per CU; 0: none)
 };
 hipError_t plain_step(int Q, int prec, int V, const StepBufs &b, bool force, bool macros_on, const FusedArgs &a,
                       hipStream_t s);
@@ -149,6 +151,8 @@ struct lbm_ctx {
     int *link_slot = nullptr;       // their force slots
     int nlink = 0;
     int V = 1;                      // nodes per lane of the fused kernel
+    bool aligned_pull = true;       // see StepBufs
+    unsigned fast_lds = 0;
 
     // force history (device + host copy)
     double *hist = nullptr;

@@ -163,7 +163,30 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_ctx **out) {
         const bool ok = c->prec == LBM_FP32 ? (v == 1 || v == 2) : c->prec != LBM_FP64 && (v == 2 || v == 4);
         if (ok) c->V = v;
     }
-    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    // 16-bit storage with two nodes per lane pulls with aligned loads and lane shifts;
+    // LBM_UNALIGNED_PULL selects the equivalent unaligned loads (same results, for comparison).
+    c->aligned_pull = std::getenv("LBM_UNALIGNED_PULL") == nullptr;
+    // Blocks of the packed kernel resident per CU: FP32 runs fastest at 3 (3 waves per SIMD,
+    // measured on the 512x256x256 case: 69.5 % of HBM peak against 67.8 % uncapped), the
+    // 16-bit formats uncapped.  The cap is a dynamic LDS allocation the kernel does not use.
+    // LBM_FAST_BLOCKS_PER_CU overrides it (0: no cap).
+    int bpc = c->prec == LBM_FP32 ? 3 : 0;
+    if (const char *env = std::getenv("LBM_FAST_BLOCKS_PER_CU")) bpc = std::max(0, std::atoi(env));
+    hipError_t e = hipSuccess;
+    if (bpc > 0) {
+        int dev = 0, lds = 0;
+        e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
+        if (e != hipSuccess) {
+            set_err(nullptr, LBM_ERR_HIP, std::string("hipDeviceGetAttribute: ") + hipGetErrorString(e));
+            delete c;
+            return LBM_ERR_HIP;
+        }
+        // midway between lds/(bpc+1) and lds/bpc, so allocation granularity cannot tip the count
+        const unsigned per = (unsigned)(2LL * lds / (2 * bpc + 1)) / 1024u * 1024u;
+        if (per > (unsigned)lds / (unsigned)(bpc + 1) && per <= 65536u) c->fast_lds = per;
+    }
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         set_err(nullptr, LBM_ERR_HIP, std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
         delete c;
@@ -586,6 +609,8 @@ StepBufs step_bufs(const lbm_ctx *c, const void *A, void *B) {
     b.block_slot = c->block_slot;
     b.partials = c->partials;
     b.macros = c->macros;
+    b.aligned_pull = c->aligned_pull;
+    b.fast_lds = c->fast_lds;
     return b;
 }
 
