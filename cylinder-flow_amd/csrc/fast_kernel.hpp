@@ -360,6 +360,17 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// The copies of k_link_v without the force, one lane per link: pair j holds the element
+// indices (source, destination) of one bounced value; every destination is a distinct solid
+// slot and no source is one, so the lanes are independent.
+template <class T>
+__global__ void __launch_bounds__(256) k_link_pairs(T *__restrict__ A, const uint32_t *__restrict__ pairs, int n) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint2 p = reinterpret_cast<const uint2 *>(pairs)[j];
+    A[p.y] = A[p.x];
+}
+
 // The per-node recomputation of the listed nodes (padded-layout indices, row-major order),
 // after k_fast_v on the same stream: the general pull of fused_common.hpp (walls, inlet,
 // outlet, periodic axes, mask bounce-back taken from the node's link word: bit i set when
@@ -419,7 +430,11 @@ static hipError_t fast_launch(const StepBufs &b, const FusedArgs &a, hipStream_t
     const Shape sh = lane_shape(a.g, V);
     const PlaneTab tab = plane_tab<Q, T>(a.g);
     hipError_t e;
-    if (b.nlink > 0) {
+    if (b.nlink > 0 && !FORCE && b.npair > 0) {
+        hipLaunchKernelGGL((k_link_pairs<T>), dim3((unsigned)((b.npair + 255) / 256)), dim3(256), 0, s, (T *)b.A,
+                           b.lpair, b.npair);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    } else if (b.nlink > 0) {
         hipLaunchKernelGGL((k_link_v<Q, S, FORCE>), dim3((unsigned)((b.nlink + 255) / 256)), dim3(256), 0, s,
                            (T *)b.A, b.link, b.linklk, b.nlink, b.link_slot, b.partials, a);
         if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -55,6 +55,8 @@ struct StepBufs {
     const uint32_t *linklk;     // their link words
     const int *link_slot;       // their force partial slots
     int nlink;
+    const uint32_t *lpair;      // the same copies one per link: (source, destination) element index pairs
+    int npair;
     const void *sfrac;          // solid fraction (FP64: double, else float) or null
     const double *uin;          // inlet velocity per (y, z)
     const int *block_slot;      // force partial slot per block (FP64) or per listed node (-1: none)
@@ -149,6 +151,8 @@ struct lbm_ctx {
     uint32_t *link = nullptr;       // FP32/16-bit: nodes whose mask links are copied before the packed path
     uint32_t *linklk = nullptr;     // their link words
     int *link_slot = nullptr;       // their force slots
+    uint32_t *lpair = nullptr;      // one (source, destination) element index pair per link
+    int npair = 0;
     int nlink = 0;
     int V = 1;                      // nodes per lane of the fused kernel
     bool aligned_pull = true;       // see StepBufs

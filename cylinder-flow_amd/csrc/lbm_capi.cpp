@@ -90,7 +90,7 @@ extern "C" const char *lbm_last_error(const lbm_ctx *ctx) { return ctx ? ctx->er
 
 static void free_all(lbm_ctx *c) {
     void *ptrs[] = {c->A, c->B, c->flags, c->dsfrac, c->duin, c->macros, c->block_slot, c->partials, c->hist, c->fix,
-                    c->fixlk, c->link, c->linklk, c->link_slot, c->rf, c->rft, c->rmac, c->rprof, c->rusr, c->rfpre};
+                    c->fixlk, c->link, c->linklk, c->link_slot, c->lpair, c->rf, c->rft, c->rmac, c->rprof, c->rusr, c->rfpre};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c->A = c->B = nullptr;
@@ -100,6 +100,8 @@ static void free_all(lbm_ctx *c) {
     c->block_slot = nullptr;
     c->fix = c->fixlk = c->link = c->linklk = nullptr;
     c->link_slot = nullptr;
+    c->lpair = nullptr;
+    c->npair = 0;
     c->nfix = c->nlink = 0;
     c->rf = c->rft = c->rmac = c->rprof = c->rfpre = nullptr;
     c->rusr = nullptr;
@@ -368,8 +370,8 @@ static int init_plain(lbm_ctx *c) {
     }
     if (!c->uin_set) inlet_profile(c, c->uin);
     // node flags (padded layout); padding columns are marked solid
-    std::vector<int> e(3 * Q);
-    lbm_lattice_table(Q, e.data(), nullptr, nullptr);
+    std::vector<int> e(3 * Q), opp(Q);
+    lbm_lattice_table(Q, e.data(), nullptr, opp.data());
     std::vector<uint8_t> flags(g.plane, FL_SOLID);
     auto solid_at = [&](int x, int y, int z) -> bool {
         if (g.xper) x = (x + g.nx) % g.nx;
@@ -485,6 +487,32 @@ static int init_plain(lbm_ctx *c) {
             HIP_TRY(c, hipMemcpy(c->linklk, linklk.data(), sizeof(uint32_t) * linklk.size(), hipMemcpyHostToDevice));
             HIP_TRY(c, hipMalloc(&c->link_slot, sizeof(int) * lslot.size()));
             HIP_TRY(c, hipMemcpy(c->link_slot, lslot.data(), sizeof(int) * lslot.size(), hipMemcpyHostToDevice));
+        }
+        // The copies again, one entry per link (steps without force): element indices of the
+        // bounced value h_opp(i)(x) and of the solid slot (x - e_i, i), sorted by the
+        // destination, so a wave's lanes share direction planes and rows.
+        if (c->nlink && (uint64_t)Q * g.plane <= UINT32_MAX) {
+            std::vector<std::pair<uint32_t, uint32_t>> pairs;   // (destination, source)
+            for (size_t j = 0; j < link.size(); j++) {
+                const uint32_t node = link[j], row = node / (uint32_t)g.nxp;
+                const int x = (int)(node - row * (uint32_t)g.nxp), y = (int)(row % (uint32_t)g.ny), z = (int)(row / (uint32_t)g.ny);
+                for (int i = 1; i < Q; i++) {
+                    if (!((linklk[j] >> i) & 1u)) continue;
+                    const int sy = (y - e[3 * i + 1] + g.ny) % g.ny, sz = (z - e[3 * i + 2] + g.nz) % g.nz;
+                    const size_t src = (size_t)(x - e[3 * i]) + (size_t)g.nxp * ((size_t)sy + (size_t)g.ny * sz);
+                    pairs.emplace_back((uint32_t)((size_t)i * g.plane + src), (uint32_t)((size_t)opp[i] * g.plane + node));
+                }
+            }
+            std::sort(pairs.begin(), pairs.end());
+            if (pairs.size() > (size_t)INT32_MAX) return set_err(c, LBM_ERR_INVALID, "too many mask links");
+            std::vector<uint32_t> flat(2 * pairs.size());
+            for (size_t k = 0; k < pairs.size(); k++) {
+                flat[2 * k] = pairs[k].second;
+                flat[2 * k + 1] = pairs[k].first;
+            }
+            c->npair = (int)pairs.size();
+            HIP_TRY(c, hipMalloc(&c->lpair, sizeof(uint32_t) * flat.size()));
+            HIP_TRY(c, hipMemcpy(c->lpair, flat.data(), sizeof(uint32_t) * flat.size(), hipMemcpyHostToDevice));
         }
     }
     const size_t esz = c->prec == LBM_FP64 ? 8 : c->prec == LBM_FP32 ? 4 : 2;
@@ -603,6 +631,8 @@ StepBufs step_bufs(const lbm_ctx *c, const void *A, void *B) {
     b.link = c->link;
     b.linklk = c->linklk;
     b.link_slot = c->link_slot;
+    b.lpair = c->lpair;
+    b.npair = c->npair;
     b.nlink = c->nlink;
     b.sfrac = c->dsfrac;
     b.uin = c->duin;
