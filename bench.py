@@ -7,9 +7,10 @@ z-spanning cylinder D=40 at (128,128), Re=300, u=0.05, half-way bounce-back on t
 four side walls, Zou-He inlet (regularised), zero-gradient outlet, drag/lift every 10 steps.  Synthetic
 initial state (seeded perturbation); the populations are resident in HBM before timing starts.
 
-A "step" is one fused lattice update of the whole domain: k_link_v (bounce-back copies into the
-solid slots), k_fast_v (packed pull-collide-store over every node) and, when its list is not
-empty, k_fix_v, back to back on the solver's stream.  MLUPS = nodes * steps / seconds over
+A "step" is one fused lattice update of the whole domain: the bounce-back copies into the solid
+slots (k_link_pairs, or k_link_v with the momentum exchange on force steps), k_fast_v (packed
+pull-collide-store over every node) and, when its list is not empty, k_fix_v, back to back on
+the solver's stream.  MLUPS = nodes * steps / seconds over
 exactly --steps steps bracketed by stream synchronisation.  The roofline uses the algorithmic
 2*Q*sizeof(storage) bytes per node update and the step's average time measured with HIP events
 on the solver's own stream (a second context with forces off, so only the step's kernels run in
@@ -237,7 +238,7 @@ def main():
                    "force_interval": 10, "body": "z-cylinder D=40", "Re": 300},
         "roofline": {"bound": "hbm", "achieved": round(ach16, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach16 / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles("fp16s"),
-                     "kernel": f"step: k_link_v + k_fast_v<27,SF16,{info16.nodes_per_lane}>", "kernel_ms": round(t16 * 1e3, 5),
+                     "kernel": f"step: k_link_pairs + k_fast_v<27,SF16,{info16.nodes_per_lane}>", "kernel_ms": round(t16 * 1e3, 5),
                      "bytes_per_update": info16.bytes_per_update},
         "forces_finite": bool(forces.size and (abs(forces[:, 1:]) < 1e30).all()),
     }
@@ -251,7 +252,7 @@ def main():
                        "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS,
                                     "unit": "GB/s", "frac": round(ach32 / HBM_PEAK_GBS, 4),
                                     "traffic": traffic_from_profiles("fp32"),
-                                    "kernel": f"step: k_link_v + k_fast_v<27,SF32,{info32.nodes_per_lane}>",
+                                    "kernel": f"step: k_link_pairs + k_fast_v<27,SF32,{info32.nodes_per_lane}>",
                                     "kernel_ms": round(t32 * 1e3, 5), "bytes_per_update": info32.bytes_per_update}}
     if not a.no_adjoint:
         log("adjoint leg")

@@ -10,10 +10,11 @@
 //    (aligned_pull), so a wave's request spans two cache lines instead of three.  It applies
 //    walls, periodic y/z, the inlet and the outlet, but knows nothing of the mask: solid nodes
 //    get values that no fluid node ever reads;
-//  * before it, k_link_v copies, for every mask link of a fluid node x (direction i pulls from
-//    the solid node s = x - e_i), the stored h_opp(i)(x) into slot (s, i) of the input buffer:
-//    the packed path's plain pull then performs the half-way bounce-back itself, and the
-//    copied values give the drag/lift;
+//  * before it, k_link_pairs (one lane per link) or, on steps that record the force, k_link_v
+//    (one lane per node, summing the momentum exchange) copies, for every mask link of a fluid
+//    node x (direction i pulls from the solid node s = x - e_i), the stored h_opp(i)(x) into
+//    slot (s, i) of the input buffer: the packed path's plain pull then performs the half-way
+//    bounce-back itself, and the copied values give the drag/lift;
 //  * after it, k_fix_v recomputes, with the per-node pull, the few nodes the packed path still
 //    gets wrong (a compact list built once from the mask: nodes with a solid fraction, the
 //    periodic x edges, the last node of a lane that reaches into the padding, mask neighbours
