@@ -37,6 +37,12 @@ PLAIN_CASES = {
                                                        periodic=(False, True, True)), 40),
     "d3q27_fully_periodic": (27, 24, 20, 18, dict(body=1, center=(12.0, 10.0, 9.0), r2=9.0,
                                                    periodic=(True, True, True)), 30),
+    # ragged and minimal extents: odd nx (the last lane reaches into the padding), thin
+    # channels, the smallest accepted box
+    "d3q27_odd_sizes_sphere": (27, 37, 13, 11, dict(body=1, center=(11.0, 6.3, 5.1), r2=9.0, inlet=1), 30),
+    "d2q9_odd_thin_disk": (9, 101, 7, 1, dict(body=1, center=(30.0, 3.2, 0.0), r2=2.0), 50),
+    "d3q27_minimal_3x3x3": (27, 3, 3, 3, dict(), 20),
+    "d2q9_minimal_3x3_periodic_y": (9, 3, 3, 1, dict(periodic=(False, True, False)), 20),
 }
 
 
@@ -188,6 +194,9 @@ def test_adjoint_matches_cpu_adjoint(pkg, orc, Q, shape, prec, tol):
     (9, (1, 30, 200), dict(body=1, center=(40.0, 14.5, 0.0), r2=25.0)),
     (27, (8, 10, 256), dict(body=3, mask="edges")),
     (9, (1, 16, 128), dict(body=3, mask="edges")),
+    (27, (5, 7, 37), dict(body=1, center=(11.0, 3.5, 2.5), r2=4.0)),
+    (9, (1, 9, 101), dict(body=1, center=(30.0, 4.0, 0.0), r2=4.0, periodic=(True, False, False))),
+    (27, (3, 3, 3), dict()),
 ])
 def test_fast_path_matches_per_node_path(pkg, orc, monkeypatch, prec, V, Q, shape, kw):
     """The packed path (V nodes per lane) followed by the per-node recomputation of the nodes
