@@ -194,6 +194,31 @@ def copy_calibration(n_bytes=1 << 30, reps=3):
     return {"bytes_read": n_bytes, "bytes_written": n_bytes, "reps": reps}
 
 
+def default_configs(pkg, steps=100):
+    """Throughput of the reference's own default configurations (lbm_config_default: the 3D
+    input.in case, D3Q27 512x128x128 sphere, and the 2D MPI case, D2Q9 512x128 disk with drag
+    every step) in reference mode (FP64, the reference's boundary rules) and in plain mode at
+    each storage precision.  HIP-event time of `steps` steps after 10 warm-up steps."""
+    L = pkg.lbm
+    rows = []
+    for Q in (27, 9):
+        c = L.default_config(Q)
+        for prec, bc, name in ((L.FP64, L.BC_REFERENCE, "reference fp64"), (L.FP64, L.BC_PLAIN, "plain fp64"),
+                               (L.FP32, L.BC_PLAIN, "plain fp32"), (L.FP16S, L.BC_PLAIN, "plain fp16s")):
+            c.precision, c.bc_mode = prec, bc
+            with L.Solver(c) as s:
+                s.initialize()
+                s.step_timed(10)
+                ms = s.step_timed(steps)
+                s.sync()
+            n = c.nx * c.ny * (c.nz if Q == 27 else 1)
+            rows.append({"lattice": f"D{3 if Q == 27 else 2}Q{Q}", "extent": [c.nx, c.ny, c.nz if Q == 27 else 1],
+                         "mode": name, "force_interval": c.force_interval, "ms_per_step": round(ms / steps, 5),
+                         "mlups": round(n * steps / ms / 1e3, 1)})
+            log(f"{rows[-1]['lattice']} {name}: {rows[-1]['mlups']} MLUPS")
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -209,6 +234,8 @@ def main():
                     help="instead of timing, compare FP32/FP16/BF16 storage with FP64 over STEPS steps")
     ap.add_argument("--accuracy-early", type=int, default=2000)
     ap.add_argument("--accuracy-window", type=int, default=None, help="first step of the statistics window")
+    ap.add_argument("--default-configs", action="store_true",
+                    help="instead of the benchmark, time the reference's default 3D and 2D configurations")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the whole domain runs on a single MI355X; "
@@ -219,6 +246,9 @@ def main():
     K, Wm = a.steps, a.warmup
     if a.accuracy:
         print(json.dumps({"accuracy": accuracy(pkg, a.accuracy, a.accuracy_early, a.accuracy_window)}), flush=True)
+        return
+    if a.default_configs:
+        print(json.dumps({"default_configs": default_configs(pkg)}), flush=True)
         return
 
     log(f"fp16s leg: {K} steps after {Wm} warm-up")

@@ -168,7 +168,6 @@ struct lbm_ctx {
     uint8_t *rusr = nullptr;
 
     bool initialized = false;
-    bool macros_valid = false;
     long step = 0;
 
     SnapshotJob *snap = nullptr;

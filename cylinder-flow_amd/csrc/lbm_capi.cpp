@@ -614,7 +614,6 @@ extern "C" int lbm_initialize(lbm_ctx *c) {
         return rc;
     }
     c->initialized = true;
-    c->macros_valid = true;
     c->step = 0;
     c->nrec = c->nrec_checked = 0;
     return LBM_OK;
@@ -656,8 +655,8 @@ static int step_n(lbm_ctx *c, int n) {
             HIP_TRY(c, ref_fills(c->rg, c->rf, c->stream));
             HIP_TRY(c, ref_usrbc(c->rg, c->rf, c->rusr, c->stream));
             if (force) HIP_TRY(c, ref_mem(c->rg, c->rf, c->rusr, c->partials, c->nslots, c->hist, c->nrec++, c->step, c->stream));
-            HIP_TRY(c, ref_fused(c->rg, c->rf, c->rft, c->rmac, c->rusr, c->rprof, k == n - 1 ? c->rfpre : nullptr,
-                                 c->stream));
+            HIP_TRY(c, ref_fused(c->rg, c->rf, c->rft, k == n - 1 ? c->rmac : nullptr, c->rusr, c->rprof,
+                                 k == n - 1 ? c->rfpre : nullptr, c->stream));
             std::swap(c->rf, c->rft);
         }
         return LBM_OK;

@@ -265,10 +265,12 @@ __global__ void __launch_bounds__(256) k_ref_fused(RefGeo r, const double *__res
         w = 0.0;
     }
     if (i == r.lx - g - 1) p = 0.0;
-    mac[n] = p;
-    mac[r.plane + n] = u;
-    mac[2 * r.plane + n] = v;
-    mac[3 * r.plane + n] = w;
+    if (mac) {   // the fields are only read after a call's last step
+        mac[n] = p;
+        mac[r.plane + n] = u;
+        mac[2 * r.plane + n] = v;
+        mac[3 * r.plane + n] = w;
+    }
     static_for<0, Q>([&](auto I) {
         constexpr int ii = I;
         const double feq = ref_feq<Q, ii>(p, u, v, w);
