@@ -89,6 +89,7 @@ struct RefGeo {
     int lx, ly, lz;              // padded extents
     size_t plane;                // lx*ly*lz
     double tau0, u0;
+    int variant;                 // D2Q9: 0 the MPI variant, 1 the single-node variant
 };
 hipError_t ref_init(const RefGeo &r, double *f, double *ft, double *mac, uint8_t *usr, const int32_t *ran,
                     int body, double cx, double cy, double cz, double r2, uint64_t seed, hipStream_t s);

@@ -16,6 +16,8 @@ using namespace lbm;
 
 static thread_local std::string t_create_err;
 
+static bool ref_mode(int bc) { return bc == LBM_BC_REFERENCE || bc == LBM_BC_REFERENCE_SN; }
+
 static int set_err(lbm_ctx *c, int code, const std::string &m) {
     if (c) c->err = m;
     else t_create_err = m;
@@ -119,13 +121,15 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_ctx **out) {
         return set_err(nullptr, LBM_ERR_INVALID, "tau0 must be > 0 (relaxation time tau0 + 0.5 > 0.5)");
     if (cfg->precision < LBM_FP64 || cfg->precision > LBM_BF16S)
         return set_err(nullptr, LBM_ERR_INVALID, "unknown precision");
-    if (cfg->bc_mode != LBM_BC_REFERENCE && cfg->bc_mode != LBM_BC_PLAIN)
+    if (!ref_mode(cfg->bc_mode) && cfg->bc_mode != LBM_BC_PLAIN)
         return set_err(nullptr, LBM_ERR_INVALID, "unknown bc_mode");
-    if (cfg->bc_mode == LBM_BC_REFERENCE && cfg->precision != LBM_FP64)
+    if (cfg->bc_mode == LBM_BC_REFERENCE_SN && Q != 9)
+        return set_err(nullptr, LBM_ERR_INVALID, "LBM_BC_REFERENCE_SN is the single-node 2D variant: lattice 9 only");
+    if (ref_mode(cfg->bc_mode) && cfg->precision != LBM_FP64)
         return set_err(nullptr, LBM_ERR_INVALID, "reference boundary mode runs in FP64 only");
     if (cfg->body < LBM_BODY_NONE || cfg->body > LBM_BODY_MASK)
         return set_err(nullptr, LBM_ERR_INVALID, "unknown body");
-    if (cfg->bc_mode == LBM_BC_REFERENCE && cfg->body == LBM_BODY_MASK)
+    if (ref_mode(cfg->bc_mode) && cfg->body == LBM_BODY_MASK)
         return set_err(nullptr, LBM_ERR_INVALID, "reference mode takes its body from the sphere/cylinder formula");
     if (cfg->inlet < LBM_INLET_UNIFORM || cfg->inlet > LBM_INLET_PROFILE)
         return set_err(nullptr, LBM_ERR_INVALID, "unknown inlet");
@@ -255,12 +259,13 @@ static void ref_extents(const lbm_ctx *c, RefGeo &r) {
     r.plane = (size_t)r.lx * r.ly * r.lz;
     r.tau0 = c->cfg.tau0;
     r.u0 = c->cfg.u0;
+    r.variant = c->cfg.bc_mode == LBM_BC_REFERENCE_SN ? 1 : 0;
 }
 
 extern "C" int lbm_set_ran(lbm_ctx *c, const int32_t *ran, size_t count) {
     if (!c || !ran) return set_err(c, LBM_ERR_INVALID, "null argument");
     if (c->initialized) return set_err(c, LBM_ERR_STATE, "lbm_set_ran after lbm_initialize");
-    if (c->cfg.bc_mode != LBM_BC_REFERENCE) return set_err(c, LBM_ERR_INVALID, "ran is the reference-mode perturbation");
+    if (!ref_mode(c->cfg.bc_mode)) return set_err(c, LBM_ERR_INVALID, "ran is the reference-mode perturbation");
     RefGeo r;
     ref_extents(c, r);
     if (count != r.plane) return set_err(c, LBM_ERR_INVALID, "ran must cover the ghost-padded box lx*ly*lz");
@@ -648,7 +653,7 @@ static int step_n(lbm_ctx *c, int n) {
     const int nforce = fi > 0 ? (int)((c->step + n) / fi - c->step / fi) : 0;
     int rc = grow_hist(c, c->nrec + nforce + 1);
     if (rc != LBM_OK) return rc;
-    if (c->cfg.bc_mode == LBM_BC_REFERENCE) {
+    if (ref_mode(c->cfg.bc_mode)) {
         for (int k = 0; k < n; k++) {
             c->step++;
             const bool force = fi > 0 && c->step % fi == 0;
@@ -734,7 +739,7 @@ extern "C" int lbm_get_fields(lbm_ctx *c, double *p, double *u, double *v, doubl
     if (!c) return LBM_ERR_INVALID;
     if (!c->initialized) return set_err(c, LBM_ERR_STATE, "lbm_get_fields before lbm_initialize");
     double *outs[4] = {p, u, v, w};
-    if (c->cfg.bc_mode == LBM_BC_REFERENCE) {
+    if (ref_mode(c->cfg.bc_mode)) {
         std::vector<double> pad(c->rg.plane);
         for (int k = 0; k < 4; k++) {
             if (!outs[k]) continue;
@@ -764,7 +769,7 @@ extern "C" int lbm_get_fields(lbm_ctx *c, double *p, double *u, double *v, doubl
 extern "C" int lbm_get_populations(lbm_ctx *c, double *f) {
     if (!c || !f) return set_err(c, LBM_ERR_INVALID, "null argument");
     if (!c->initialized) return set_err(c, LBM_ERR_STATE, "lbm_get_populations before lbm_initialize");
-    if (c->cfg.bc_mode == LBM_BC_REFERENCE) {
+    if (ref_mode(c->cfg.bc_mode)) {
         std::vector<double> pad(c->rg.plane);
         for (int i = 0; i < c->Q; i++) {
             HIP_TRY(c, hipMemcpyAsync(pad.data(), c->rfpre + (size_t)i * c->rg.plane, sizeof(double) * c->rg.plane,
@@ -793,7 +798,7 @@ extern "C" int lbm_get_populations(lbm_ctx *c, double *f) {
 extern "C" int lbm_set_populations(lbm_ctx *c, const double *f) {
     if (!c || !f) return set_err(c, LBM_ERR_INVALID, "null argument");
     if (!c->initialized) return set_err(c, LBM_ERR_STATE, "lbm_set_populations before lbm_initialize");
-    if (c->cfg.bc_mode == LBM_BC_REFERENCE)
+    if (ref_mode(c->cfg.bc_mode))
         return set_err(c, LBM_ERR_INVALID, "reference mode keeps ghost state; populations cannot be replaced");
     const FusedArgs a = ctx_fused_args(c);
     double *tmp = nullptr;

@@ -74,6 +74,7 @@ __global__ void __launch_bounds__(256) k_ref_init(RefGeo r, RefBody b, double *_
     usr[n] = (uint8_t)us;
     double u;
     if constexpr (L::D == 3) u = ref_profile(r.u0, z, r.glz) * (double)us * (1.0 + (double)ran * 0.001);
+    else if (r.variant == 1) u = ref_profile(r.u0, y, r.gly) * (double)us * (1.0 + (double)ran * 0.001);   // Kokkos/lbm.cpp:123
     else u = ref_profile(r.u0, y, r.gly) * (double)us * (0.9 + (double)ran * 0.002);
     mac[n] = 0.0;
     mac[r.plane + n] = u;
@@ -138,6 +139,7 @@ __global__ void __launch_bounds__(256) k_ref_fill2(RefGeo r, double *__restrict_
     } else {
         if (t >= lx - 1) return;
         const int i = t;
+        if (r.variant == 1 && (i < 2 || i >= lx - 2)) return;   // Kokkos/lbm.cpp:158: i in [2, lx-2)
         if (FACE == 2) {
             f[RIDX(r, 2, i, 1, 0)] = f[RIDX(r, 4, i, 3, 0)];
             f[RIDX(r, 5, i, 1, 0)] = f[RIDX(r, 7, i + 2, 3, 0)];

@@ -157,7 +157,7 @@ extern "C" int lbm_snapshot_async(lbm_ctx *c, const char *path, int n) {
     }
     int rc = snapshot_wait(c);
     if (rc != LBM_OK) return rc;
-    const bool ref = c->cfg.bc_mode == LBM_BC_REFERENCE;
+    const bool ref = c->cfg.bc_mode != LBM_BC_PLAIN;
     const size_t plane = ref ? c->rg.plane : c->N;
     double *pinned = nullptr, *dtmp = nullptr;
     hipEvent_t ev = nullptr;

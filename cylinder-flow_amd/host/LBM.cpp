@@ -7,7 +7,8 @@
 #include <iostream>
 
 LBM::LBM(int sx, int sy, int sz, double &tau, double &rho0_, double &u0_, const LBMOptions &o)
-    : glx(sx), gly(sy), glz(o.lattice == 9 ? 1 : sz), rho0(rho0_), tau0(tau), u0(u0_), opt(o) {
+    : glx(o.single_node ? sx + 1 : sx), gly(o.single_node ? sy + 1 : sy), glz(o.lattice == 9 ? 1 : sz), rho0(rho0_),
+      tau0(tau), u0(u0_), opt(o) {
     check(lbm_config_default(&cfg, o.lattice), "lbm_config_default");
     cfg.nx = glx;
     cfg.ny = gly;
@@ -27,11 +28,19 @@ LBM::LBM(int sx, int sy, int sz, double &tau, double &rho0_, double &u0_, const 
         cfg.cx = cfg.cy = cfg.cz = (double)((glx - 1) / 8);
         const double r = (glx - 1) / 64.0;
         cfg.r2 = r * r;
+    } else if (o.single_node) {     // Kokkos/lbm.cpp:115 in padded coordinates (node x = i - 2)
+        cfg.cx = 0.25 * sx - 2.0;
+        cfg.cy = 0.5 * sy;
+        cfg.r2 = 0.01 * sy * sy;
     } else {                        // MPI Kokkos/lbm.cpp:108
         cfg.cx = 0.5 * (glx - 1);
         cfg.cy = 0.5 * (gly - 1);
         const double r = 0.03125 * (glx - 1) / 2.0;
         cfg.r2 = r * r;
+    }
+    if (o.single_node) {
+        cfg.bc_mode = LBM_BC_REFERENCE_SN;
+        cfg.force_interval = 0;     // the single-node variant computes no drag
     }
     cfg.inlet = o.inlet;
     cfg.perturb = o.perturb;
@@ -104,7 +113,9 @@ void LBM::MPIoutput(int n) {
 // MPI Kokkos/lbm.cpp:738-765 (x y u v p)
 void LBM::Output(int n) {
     check(lbm_snapshot_wait(ctx), "lbm_snapshot_wait");
-    const std::string name = "output" + std::to_string(n) + "0.dat";
+    // the single-node 2D variant writes output<n>.dat, x scaled by 4 (Kokkos/lbm.cpp:219-243)
+    const std::string name = "output" + std::to_string(n) + (opt.single_node ? ".dat" : "0.dat");
+    const double xs = opt.single_node ? 4.0 : 1.0;
     std::ofstream out;
     out << std::setiosflags(std::ios::fixed);
     out.open(name, std::ios::out);
@@ -128,7 +139,7 @@ void LBM::Output(int n) {
         size_t o = 0;
         for (int j = 0; j < gly; j++)
             for (int i = 0; i < glx; i++, o++)
-                out << std::setprecision(8) << std::setiosflags(std::ios::left) << i / (glx - 1.0) << " "
+                out << std::setprecision(8) << std::setiosflags(std::ios::left) << xs * i / (glx - 1.0) << " "
                     << j / (gly - 1.0) << " " << u[o] << " " << v[o] << " " << p[o] << std::endl;
     }
     std::printf("\nThe result %d is writen\n\n============================\n", n);

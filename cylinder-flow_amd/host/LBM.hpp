@@ -11,6 +11,7 @@
 //   Boundary()                   no-op (declared but never defined in the reference)
 //   MPIoutput(n)                 binary Tecplot "output<n>.plt", written asynchronously
 //   Output(n)                    ASCII "output<n><rank>.dat" as the reference writes it
+//                                (single-node 2D variant: "output<n>.dat", Kokkos/lbm.cpp:219-243)
 // Device data are owned by the context; fields()/populations() copy them to the host (the
 // reference exposes UVM views instead).  Cd/Cl are appended to test.dat like the 2D MPI
 // variant (MPI Kokkos/lbm.cpp:398-407) when force logging is on.
@@ -35,6 +36,8 @@ struct LBMOptions {
     int force_interval = -1;      // -1: the reference's default (2D every step, 3D none)
     double cd_length = 0.0;       // Cd = 2F/(u0^2 * cd_length); 0: the reference's 0.03125*(glx-1)
     bool log_test_dat = false;
+    bool single_node = false;     // 2D single-node variant: LBM(sx, sy, ...) holds sx+1 by sy+1
+                                  // nodes (Kokkos/lbm.hpp:26), its disk and boundary rules
 };
 
 struct LBM {

@@ -53,7 +53,7 @@ void System::parse(const std::string &path) {
 
 System::System(int &nx, int &ny) : sx(nx), sy(ny), sz(1), dim(2) {}
 
-// MPI Kokkos/System.cpp:3-18
+// MPI Kokkos/System.cpp:3-18 ; single node: Kokkos/System.cpp:3-18 (R is an int in both)
 void System::Initialize() {
     cs2 = 1.0 / 3.0;
     cs = std::sqrt(cs2);
@@ -63,8 +63,8 @@ void System::Initialize() {
     u0 = 0.1;
     Ma = u0 / cs;
     miu = rho0 * u0 * R / Re;
-    Time = 1000;
-    inter = 100;
+    Time = single_node ? 100000 : 1000;
+    inter = single_node ? 10000 : 100;
     tau = u0 * R / Re / cs2;
     parsed_ok = true;
 }

@@ -5,6 +5,9 @@
 //   Streaming/Update with MPIoutput and the elapsed time every `inter` steps.
 // 2D (--2d): MPI Kokkos/main.cpp:6-46 -- nx = 512, ny = 128, hard-coded parameters,
 //   Output(0) and Output every `inter` steps, Cd/Cl appended to test.dat every step.
+// 2D single node (--2d-single): Kokkos/main.cpp:6-45 -- nx = 512, ny = 128, the single-node
+//   variant's parameters (Time 100000, inter 10000), sx+1 by sy+1 nodes, its wall and inlet
+//   fill order, no drag, Output(0) and Output every `inter` steps.
 // Extra flags (not in the reference): --input PATH, --time N (cap the step count),
 //   --check (parse and print the parameters, no GPU work).
 #include <chrono>
@@ -52,14 +55,15 @@ static LBMOptions options_from(const System &s) {
 int main(int argc, char **argv) {
     std::string input = "input.in";
     long cap = -1;
-    bool check = false, two_d = false;
+    bool check = false, two_d = false, single = false;
     for (int a = 1; a < argc; a++) {
         if (!std::strcmp(argv[a], "--input") && a + 1 < argc) input = argv[++a];
         else if (!std::strcmp(argv[a], "--time") && a + 1 < argc) cap = std::atol(argv[++a]);
         else if (!std::strcmp(argv[a], "--check")) check = true;
         else if (!std::strcmp(argv[a], "--2d")) two_d = true;
+        else if (!std::strcmp(argv[a], "--2d-single")) two_d = single = true;
         else {
-            std::fprintf(stderr, "usage: %s [--input input.in] [--time N] [--check] [--2d]\n", argv[0]);
+            std::fprintf(stderr, "usage: %s [--input input.in] [--time N] [--check] [--2d | --2d-single]\n", argv[0]);
             return 2;
         }
     }
@@ -67,12 +71,14 @@ int main(int argc, char **argv) {
         if (two_d) {
             int nx = 512, ny = 128;
             System s1(nx, ny);
+            s1.single_node = single;
             s1.Initialize();
             s1.Monitor();
             if (check) return 0;
             LBMOptions o;
             o.lattice = 9;
-            o.log_test_dat = true;
+            o.single_node = single;
+            o.log_test_dat = !single;
             double tau = s1.tau, rho0 = s1.rho0, u0 = s1.u0;
             LBM l1(0, s1.sx, s1.sy, 1, tau, rho0, u0, o);
             l1.Initialize();

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblbm_hip.so")
 
 FP64, FP32, FP16S, BF16S = 0, 1, 2, 3
-BC_REFERENCE, BC_PLAIN = 0, 1
+BC_REFERENCE, BC_PLAIN, BC_REFERENCE_SN = 0, 1, 2
 BODY_NONE, BODY_SPHERE, BODY_ZCYL, BODY_MASK = 0, 1, 2, 3
 INLET_UNIFORM, INLET_PARABOLIC, INLET_LEGENDRE, INLET_PROFILE = 0, 1, 2, 3
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "fp16s": FP16S, "bf16s": BF16S}

@@ -39,10 +39,14 @@ extern "C" {
 enum lbm_precision { LBM_FP64 = 0, LBM_FP32 = 1, LBM_FP16S = 2, LBM_BF16S = 3 };
 /* LBM_BC_REFERENCE: the reference's own boundary rules and kernel order (ghost layers,
  * link-mirror walls and body, ghost-copy inlet with prescribed-velocity relaxation,
- * lateral outlet extrapolation with p = 0); FP64 only.
+ * lateral outlet extrapolation with p = 0); FP64 only.  D2Q9 follows the MPI 2D variant
+ * (MPI Kokkos/lbm.cpp).
+ * LBM_BC_REFERENCE_SN: the same for the single-node 2D variant (Kokkos/lbm.cpp:53-218), D2Q9
+ * and FP64 only: initial perturbation 1 + ran*0.001 and bottom/top wall fills over the
+ * columns i in [2, lx-2) (its two inlet-corner pulls read the previous step's wall fill).
  * LBM_BC_PLAIN: half-way bounce-back walls and body, Zou-He velocity inlet,
  * zero-gradient outlet, optional periodic axes and partial bounce-back. */
-enum lbm_bc_mode { LBM_BC_REFERENCE = 0, LBM_BC_PLAIN = 1 };
+enum lbm_bc_mode { LBM_BC_REFERENCE = 0, LBM_BC_PLAIN = 1, LBM_BC_REFERENCE_SN = 2 };
 enum lbm_body { LBM_BODY_NONE = 0, LBM_BODY_SPHERE = 1 /* disk in 2D */, LBM_BODY_ZCYL = 2,
                 LBM_BODY_MASK = 3 /* lbm_set_mask */ };
 enum lbm_inlet { LBM_INLET_UNIFORM = 0, LBM_INLET_PARABOLIC = 1, LBM_INLET_LEGENDRE = 2,

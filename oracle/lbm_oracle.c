@@ -148,6 +148,7 @@ struct orc_ref {
     int *usr, *ran;
     long step;
     int force_interval;
+    int variant;                /* 2D: 0 MPI variant, 1 single-node variant (orc_cfg.variant) */
     double *frec; int nrec, caprec;
 };
 
@@ -188,6 +189,7 @@ orc_ref *orc_ref_create(const orc_cfg *c, const int *ran_in)
     s->lz = (D == 3) ? s->glz + 2 * s->g : 1;
     s->tau0 = c->tau0; s->u0 = c->u0;
     s->force_interval = c->force_interval;
+    s->variant = D == 2 ? c->variant : 0;
     size_t n = (size_t)s->lx * s->ly * s->lz;
     s->f = (double *)calloc(n * Q, sizeof(double));
     s->ft = (double *)calloc(n * Q, sizeof(double));
@@ -214,6 +216,8 @@ orc_ref *orc_ref_create(const orc_cfg *c, const int *ran_in)
                 s->wa[L] = 0.0;
                 if (D == 3)
                     s->ua[L] = ref_profile(s->u0, z, s->glz) * (double)usr * (1.0 + (double)r * 0.001);
+                else if (s->variant == 1)   /* Kokkos/lbm.cpp:123 */
+                    s->ua[L] = ref_profile(s->u0, y, s->gly) * (double)usr * (1.0 + (double)r * 0.001);
                 else
                     s->ua[L] = ref_profile(s->u0, y, s->gly) * (double)usr * (0.9 + (double)r * 0.002);
                 for (int ii = 0; ii < Q; ii++) {
@@ -314,14 +318,17 @@ void orc_ref_boundary_fills(orc_ref *s)
             RF(s, 7, lx - 2, j, 0) = RF(s, 7, lx - 3, j - 1, 0);
             RF(s, 6, lx - 2, j, 0) = RF(s, 6, lx - 3, j + 1, 0);
         }
-        /* stream4, bottom (i-2 = -1 and i+2 = lx wrap through the flat index) */
-        for (int i = 1; i < lx - 1; i++) {
+        /* stream4, bottom (i-2 = -1 and i+2 = lx wrap through the flat index); the single-node
+         * variant fills i in [2, lx-2) only (Kokkos/lbm.cpp:158-167), leaving the corner
+         * columns to stream3 */
+        const int i0 = s->variant == 1 ? 2 : 1, i1 = s->variant == 1 ? lx - 2 : lx - 1;
+        for (int i = i0; i < i1; i++) {
             RF(s, 2, i, 1, 0) = RF(s, 4, i, 3, 0);
             RF(s, 5, i, 1, 0) = RF(s, 7, i + 2, 3, 0);
             RF(s, 6, i, 1, 0) = RF(s, 8, i - 2, 3, 0);
         }
         /* stream4, top */
-        for (int i = 1; i < lx - 1; i++) {
+        for (int i = i0; i < i1; i++) {
             RF(s, 4, i, ly - 2, 0) = RF(s, 2, i, ly - 4, 0);
             RF(s, 7, i, ly - 2, 0) = RF(s, 5, i - 2, ly - 4, 0);
             RF(s, 8, i, ly - 2, 0) = RF(s, 6, i + 2, ly - 4, 0);

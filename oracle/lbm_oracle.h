@@ -13,6 +13,7 @@
  *             [momentum exchange] -> stream1 -> stream2 -> Update.
  *               D3Q27: 3D MPI+KOKKOS+MPIIO/lbm.cpp:36-417, lbm.hpp:146-219 (ghost 3)
  *               D2Q9 : MPI Kokkos/lbm.cpp:47-139,313-457, lbm.hpp:77-85     (ghost 2)
+ *               D2Q9, single-node variant: Kokkos/lbm.cpp:53-218 (cfg.variant = 1)
  *  "plain" -- the configuration physics (no reference counterpart): half-way bounce-back
  *             on walls and on the solid mask, Zou-He velocity inlet regularised at the
  *             inlet node (Latt & Chopard's projection: plain Zou-He is unstable with BGK
@@ -52,6 +53,9 @@ typedef struct orc_cfg {
     double perturb;     /* plain mode: u_x += perturb * xi, xi in [-1,1) */
     int xper, yper, zper;   /* plain mode: periodic axes (x periodic replaces inlet/outlet) */
     int force_interval; /* record forces every n steps (0: never) */
+    int variant;        /* ref mode, D2Q9: 0 the MPI variant (MPI Kokkos/), 1 the single-node
+                           variant (Kokkos/lbm.cpp): perturbation 1 + ran*0.001 and bottom/top
+                           wall fills over i in [2, lx-2) only */
 } orc_cfg;
 
 /* ---- counter-based RNG shared (by specification, not by code) with the GPU init ---- */

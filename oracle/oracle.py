@@ -27,7 +27,7 @@ class OrcCfg(C.Structure):
         ("inlet", C.c_int), ("ncoef", C.c_int), ("coef", C.c_double * 8),
         ("seed", C.c_uint64), ("perturb", C.c_double),
         ("xper", C.c_int), ("yper", C.c_int), ("zper", C.c_int),
-        ("force_interval", C.c_int),
+        ("force_interval", C.c_int), ("variant", C.c_int),
     ]
 
 
@@ -97,7 +97,7 @@ def _ptr(a):
 
 def make_cfg(Q, nx, ny, nz=1, tau0=0.03, u0=0.05, body=BODY_NONE, center=(0.0, 0.0, 0.0),
              r2=0.0, inlet=INLET_UNIFORM, coef=(), seed=5374857, perturb=0.0,
-             periodic=(False, False, False), force_interval=0) -> OrcCfg:
+             periodic=(False, False, False), force_interval=0, variant=0) -> OrcCfg:
     c = OrcCfg()
     c.Q, c.nx, c.ny, c.nz = Q, nx, ny, (nz if Q == 27 else 1)
     c.tau0, c.u0 = tau0, u0
@@ -111,6 +111,7 @@ def make_cfg(Q, nx, ny, nz=1, tau0=0.03, u0=0.05, body=BODY_NONE, center=(0.0, 0
     c.seed, c.perturb = seed, perturb
     c.xper, c.yper, c.zper = (int(bool(p)) for p in periodic)
     c.force_interval = force_interval
+    c.variant = variant
     return c
 
 

@@ -10,6 +10,7 @@ vectors for this path, so the fixtures freeze the oracle's results on seeded inp
 oracle itself is pinned by tests/test_oracle.py.
 
     python tests/golden/make_golden.py          # rewrites every fixture
+    python tests/golden/make_golden.py NAME...  # rewrites the named ones
 """
 import json
 import os
@@ -41,6 +42,9 @@ CASES = {
                                   center=(20.0, 15.5, 0.0), r2=16.0, force_interval=1), 40, {}),
     "ref_d3q27_sphere": ("ref", dict(Q=27, nx=32, ny=16, nz=14, tau0=0.024, u0=0.1, body=O.BODY_SPHERE,
                                      center=(10.0, 7.0, 6.0), r2=9.0, force_interval=1), 20, {"ran": 17}),
+    "ref_d2q9_single_node": ("ref", dict(Q=9, nx=49, ny=21, nz=1, tau0=0.036, u0=0.1, body=O.BODY_SPHERE,
+                                         center=(0.25 * 48 - 2.0, 0.5 * 20, 0.0), r2=0.01 * 20 * 20,
+                                         force_interval=5, variant=1), 40, {"ran": 23}),
     "adjoint_d2q9_mask_sfrac": ("adjoint", dict(Q=9, nx=24, ny=12, nz=1, tau0=0.1, u0=0.05, body=O.BODY_MASK,
                                                 inlet=O.INLET_PARABOLIC, perturb=1e-3), 20,
                                 {"mask": 5, "sfrac": 6, "n0": 8, "K": 7}),
@@ -109,7 +113,7 @@ def run(name):
 
 
 def main():
-    for name in CASES:
+    for name in (sys.argv[1:] or CASES):
         meta, arrs, res = run(name)
         path = os.path.join(HERE, name + ".npz")
         np.savez_compressed(path, meta=np.array(json.dumps(meta)), **{"in_" + k: v for k, v in arrs.items()},

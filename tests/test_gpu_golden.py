@@ -15,6 +15,8 @@ def gpu_solver(pkg, meta, inp, precision, bc_mode):
     L = pkg.lbm
     kw = dict(meta["cfg"])
     Q, nx, ny, nz = kw.pop("Q"), kw.pop("nx"), kw.pop("ny"), kw.pop("nz")
+    if kw.pop("variant", 0) == 1:   # the single-node 2D variant of the reference rules
+        bc_mode = L.BC_REFERENCE_SN
     for k in ("center", "periodic", "coef"):
         if k in kw:
             kw[k] = tuple(kw[k])

@@ -126,3 +126,26 @@ def test_driver_2d_test_dat(pkg, tmp_path):
     for k, f in enumerate((u, v, p)):
         assert np.abs(a[:, 2 + k] - f.ravel()).max() <= 1e-8
     assert os.path.exists(tmp_path / "output00.dat") and os.path.exists(tmp_path / "output10.dat")
+
+
+def test_driver_2d_single_node_variant(pkg, tmp_path):
+    """--2d-single: the single-node 2D variant's main (512 x 128 -> 513 x 129 nodes); Output(0)
+    writes output0.dat with x = 4 i / (nx - 1), equal to the initial fields of the same
+    configuration through the Python binding (LBM_BC_REFERENCE_SN)."""
+    run_driver(["--2d-single", "--time", "30"], tmp_path)
+    L = pkg.lbm
+    c = L.make_config(9, 513, 129, tau0=0.1 * 12 / 100 / (1.0 / 3.0), u0=0.1, precision=L.FP64,
+                      bc_mode=L.BC_REFERENCE_SN, body=L.BODY_SPHERE, center=(0.25 * 512 - 2.0, 0.5 * 128, 0.0),
+                      r2=0.01 * 128 * 128, force_interval=0)
+    with L.Solver(c) as s:
+        s.initialize()
+        p, u, v, _ = s.fields()
+    with open(tmp_path / "output0.dat") as fh:
+        assert fh.readline().strip() == "variables=x,y,u,v,p"
+        assert fh.readline().strip() == "zone I=513,J=129"
+    a = np.loadtxt(tmp_path / "output0.dat", skiprows=2)
+    assert a.shape == (513 * 129, 5)
+    assert np.allclose(a[:513, 0], 4.0 * np.arange(513) / 512, atol=5e-9)
+    for k, f in enumerate((u, v, p)):
+        assert np.abs(a[:, 2 + k] - f.ravel()).max() <= 1e-8
+    assert not (tmp_path / "test.dat").exists() and not (tmp_path / "output00.dat").exists()

@@ -158,6 +158,36 @@ def test_reference_mode_injected_ran(pkg, orc):
     assert rel_l2(gs.fields()[1], rs.fields()[1]) <= 1e-13
 
 
+@pytest.mark.parametrize("sx,sy,steps", [(48, 20, 60), (64, 31, 40)])
+def test_single_node_2d_variant_matches_oracle(pkg, orc, sx, sy, steps):
+    """The single-node 2D variant (LBM(sx, sy) holds sx+1 by sy+1 nodes, its disk at
+    (0.25 sx, 0.5 sy) of radius 0.1 sy, perturbation 1 + ran*0.001, wall fills over
+    i in [2, lx-2)) against the oracle's restatement with the same injected ran; and it is
+    not the MPI variant: the same inputs in LBM_BC_REFERENCE give different fields."""
+    L = pkg.lbm
+    nx, ny = sx + 1, sy + 1
+    body = dict(center=(0.25 * sx - 2.0, 0.5 * sy, 0.0), r2=0.01 * sy * sy)
+    ran = np.random.default_rng(sx).integers(0, 100, (ny + 4) * (nx + 4)).astype(np.int32)
+    out = {}
+    for bc in (L.BC_REFERENCE_SN, L.BC_REFERENCE):
+        g = L.make_config(9, nx, ny, tau0=0.036, u0=0.1, precision=L.FP64, bc_mode=bc, body=L.BODY_SPHERE,
+                          force_interval=0, **body)
+        print(f"[parity] single-node 2D variant {sx}x{sy} bc={bc}, {steps} steps", flush=True)
+        with pkg.Solver(g) as gs:
+            gs.set_ran(ran)
+            gs.initialize()
+            gs.step(steps)
+            out[bc] = (np.stack(gs.fields()), gs.populations())
+    o = orc.make_cfg(9, nx, ny, tau0=0.036, u0=0.1, body=orc.BODY_SPHERE, variant=1, **body)
+    rs = orc.RefState(o, ran=ran)
+    rs.step(steps)
+    fg, pg = out[L.BC_REFERENCE_SN]
+    for a, b in zip(fg, rs.fields()):
+        assert np.abs(a - b).max() <= 1e-13 * max(1.0, np.abs(b).max())
+    assert rel_l2(pg, rs.populations()) <= 1e-13
+    assert rel_l2(out[L.BC_REFERENCE][0][1], fg[1]) > 1e-6
+
+
 @pytest.mark.parametrize("Q,shape,prec,tol", [(9, (1, 20, 40), "fp64", 1e-9), (27, (10, 10, 16), "fp64", 1e-9),
                                               (9, (1, 20, 40), "fp32", 2e-3)])
 def test_adjoint_matches_cpu_adjoint(pkg, orc, Q, shape, prec, tol):

@@ -69,8 +69,8 @@ def test_equilibrium_moments(orc, Q):
 
 
 # ------------------------------------------------------------------ reference fill order
-def _fill_provenance(orc, Q, nx, ny, nz, gens=3):
-    r = orc.RefState(orc.make_cfg(Q, nx, ny, nz, tau0=0.1, u0=0.1))
+def _fill_provenance(orc, Q, nx, ny, nz, gens=3, variant=0, where=False):
+    r = orc.RefState(orc.make_cfg(Q, nx, ny, nz, tau0=0.1, u0=0.1, variant=variant))
     lx, ly, lz, g = r.lx, r.ly, r.lz, r.ghost
     e, _, _ = orc.lattice(Q)
     ids = np.arange(lx * ly * lz * Q, dtype=np.float64)
@@ -90,6 +90,9 @@ def _fill_provenance(orc, Q, nx, ny, nz, gens=3):
         gv = (f[src.ravel()] // B).astype(int)
         counts = dict(current=int((gv == gen).sum()), lagged=int((gv == gen - 1).sum()),
                       initial=int(((gv == 0) & (gen > 1)).sum()))
+    if where:   # lagged pulls as (i, j, direction) in padded coordinates
+        lag = np.argwhere(gv.reshape(node.shape + (Q,)) == gens - 1)
+        counts["lagged_at"] = sorted((int(I[tuple(t[:-1])]), int(J[tuple(t[:-1])]), int(t[-1])) for t in lag)
     return counts
 
 
@@ -101,6 +104,36 @@ def test_reference_3d_fill_order_provenance(orc):
 def test_reference_2d_fill_order_has_no_stale_pulls(orc):
     c = _fill_provenance(orc, 9, 12, 10, 1)
     assert c["lagged"] == 0 and c["initial"] == 0
+
+
+def test_single_node_2d_variant_lags_two_inlet_corner_pulls(orc):
+    """The single-node 2D variant fills the bottom/top walls over i in [2, lx-2) only and runs
+    the inlet copy first (Kokkos/lbm.cpp:147-167), so exactly two pulls per step read the
+    previous step's wall fill: node (2, 2) direction 5 and node (2, ly-3) direction 8 (an
+    independent analysis of that fill order finds the same two)."""
+    nx, ny = 12, 10
+    c = _fill_provenance(orc, 9, nx, ny, 1, variant=1, where=True)
+    ly = ny + 4
+    assert c["lagged"] == 2 and c["initial"] == 0
+    assert c["lagged_at"] == [(2, 2, 5), (2, ly - 3, 8)]
+
+
+def test_single_node_variant_initial_velocity(orc):
+    """Initial u of the 2D variants (injected ran): the parabola times 0.9 + 0.002 ran (MPI
+    variant, MPI Kokkos/lbm.cpp:109) or 1 + 0.001 ran (single-node variant, Kokkos/lbm.cpp:123),
+    zero in the body."""
+    nx, ny = 21, 9
+    ran = np.random.default_rng(1).integers(0, 100, (ny + 4) * (nx + 4)).astype(np.int32)
+    y = np.arange(ny)[:, None]
+    prof = 0.1 * 4.0 * y * (ny - 1 - y) / (ny - 1) ** 2
+    r = ran.reshape(ny + 4, nx + 4)[2:-2, 2:-2]
+    for variant, factor in ((0, 0.9 + 0.002 * r), (1, 1.0 + 0.001 * r)):
+        s = orc.RefState(orc.make_cfg(9, nx, ny, tau0=0.036, u0=0.1, body=orc.BODY_SPHERE, center=(6.0, 4.0, 0.0),
+                                      r2=2.0, variant=variant), ran=ran)
+        u = s.fields()[1][0]
+        xx, yy = np.meshgrid(np.arange(nx), np.arange(ny))
+        solid = (xx - 6.0) ** 2 + (yy - 4.0) ** 2 <= 2.0
+        assert np.allclose(u, np.where(solid, 0.0, prof * factor), rtol=1e-14, atol=1e-17)
 
 
 def test_reference_first_collision_is_identity(orc):
