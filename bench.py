@@ -103,6 +103,31 @@ def cpu_baseline(seconds=15.0):
             "sample": f"FP64 oracle, {n} steps of a 512x256x32 slab of the workload ({dt:.1f} s)"}
 
 
+def cpu_reference_structured(seconds=10.0):
+    """The oracle's reference mode on the same 512x256x32 slab: the reference's own algorithm
+    and structure (collision, six ghost fills, usrbc, stream into ft, copy back, the serial
+    moment update; AoS ghost-padded layout; OpenMP on the collision and stream loops, the
+    update serial as in the reference), with its boundary rules in place of the workload's.
+    The nearest stand-in for the reference's CPU build, which cannot be built here."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    nx, ny, nz = 512, 256, 32
+    u, D, Re = 0.05, 40.0, 300.0
+    cfg = O.make_cfg(27, nx, ny, nz, tau0=3 * u * D / Re, u0=u, body=O.BODY_ZCYL, center=(128.0, 128.0, 0.0),
+                     r2=(D / 2) ** 2, force_interval=10)
+    st = O.RefState(cfg)
+    st.step(1)
+    t0 = time.perf_counter()
+    st.step(1)
+    one = time.perf_counter() - t0
+    n = max(2, int(seconds / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    st.step(n)
+    dt = time.perf_counter() - t0
+    return {"value": round(nx * ny * nz * n / dt / 1e6, 2), "unit": "MLUPS", "cores": O.lib().orc_num_threads(),
+            "kind": "port", "sample": f"FP64 oracle reference mode, {n} steps of a 512x256x32 slab ({dt:.1f} s)"}
+
+
 def adjoint_ratio(pkg, steps=500):
     """Adjoint (recompute + reverse sweep) over forward time on the D3Q27 adjoint workload's
     192x96x96 geometry, FP32, window [steps/2, steps], checkpoint every 250 steps."""
@@ -290,6 +315,8 @@ def main():
     if not a.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+        log("cpu reference-structured")
+        out["cpu_reference_structured"] = cpu_reference_structured(a.cpu_seconds * 2.0 / 3.0)
     if a.calibrate_copy:
         log("copy calibration")
         out["copy_calibration"] = copy_calibration()
