@@ -47,7 +47,7 @@ def log(msg):
 
 
 def run_leg(pkg, precision, steps, warmup, force_interval):
-    W = pkg.configs.cfg2_zcyl3d(precision=precision, force_interval=force_interval)
+    W = pkg.configs.zcylinder3d_512x256x256(precision=precision, force_interval=force_interval)
     s = pkg.Solver(W.cfg).initialize()
     s.step(warmup)
     s.sync()
@@ -61,7 +61,7 @@ def run_leg(pkg, precision, steps, warmup, force_interval):
 
 def kernel_roofline(pkg, precision, steps, bytes_per_update, nodes):
     """Average duration of one step's kernels alone (forces off), HIP events on their stream."""
-    W = pkg.configs.cfg2_zcyl3d(precision=precision, force_interval=0)
+    W = pkg.configs.zcylinder3d_512x256x256(precision=precision, force_interval=0)
     s = pkg.Solver(W.cfg).initialize()
     s.step(5)
     ms = s.step_timed(steps)
@@ -131,13 +131,13 @@ def cpu_reference_structured(seconds=10.0):
 def adjoint_ratio(pkg, steps=500):
     """Adjoint (recompute + reverse sweep) over forward time on the D3Q27 adjoint workload's
     192x96x96 geometry, FP32, window [steps/2, steps], checkpoint every 250 steps."""
-    W = pkg.configs.cfg4_adjoint3d()
+    W = pkg.configs.adjoint3d_192x96x96()
     s = pkg.Solver(W.cfg)
     s.set_solid_fraction(W.extra["sfrac"])
     s.initialize()
     J, ds, du, tf, ta = s.adjoint(steps // 2, steps, W.cd_scale(), 250)
     s.close()
-    return {"workload": "cfg4_d3q27_adjoint_192x96x96 (fp32), %d steps, checkpoint 250" % steps,
+    return {"workload": "d3q27_adjoint_192x96x96 (fp32), %d steps, checkpoint 250" % steps,
             "forward_ms": round(tf, 3), "adjoint_ms": round(ta, 3), "ratio": round(ta / tf, 3)}
 
 
@@ -154,7 +154,7 @@ def accuracy(pkg, steps, early, window_start=None):
     L = pkg.lbm
     out, ref = {}, None
     for name, prec in (("fp64", L.FP64), ("fp32", L.FP32), ("fp16s", L.FP16S), ("bf16s", L.BF16S)):
-        W = pkg.configs.cfg2_zcyl3d(precision=prec, force_interval=10)
+        W = pkg.configs.zcylinder3d_512x256x256(precision=prec, force_interval=10)
         s = pkg.Solver(W.cfg).initialize()
         s.step(early)
         _, u, v, w = s.fields()
@@ -187,7 +187,7 @@ def accuracy(pkg, steps, early, window_start=None):
         del u, v, w
         out[name] = rec
         log(f"accuracy {name}: {rec}")
-    out["workload"] = f"cfg2 512x256x256 Re 300, {steps} steps, statistics over steps {ws}-{steps}, " \
+    out["workload"] = f"D3Q27 z-cylinder 512x256x256 Re 300, {steps} steps, statistics over steps {ws}-{steps}, " \
                       f"velocity compared at step {early}"
     return out
 
@@ -264,7 +264,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the whole domain runs on a single MI355X; "
-              "multi-GPU runs are not part of this build (use --gpus 1)", file=sys.stderr)
+              "multi-GPU runs are not supported by this program (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
     pkg = load_package()
     L = pkg.lbm

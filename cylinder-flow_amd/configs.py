@@ -35,31 +35,31 @@ class Workload:
         return 2.0 / (self.u0 * self.u0 * self.area)
 
 
-def cfg0_cylinder2d(precision=L.FP64, **kw) -> Workload:
+def cylinder2d_400x100(precision=L.FP64, **kw) -> Workload:
     """D2Q9 400x100, cylinder D=20 at (80,50), Re=100, u=0.05, 20 000 steps, forces every 10."""
     u, D, Re = 0.05, 20.0, 100.0
     c = L.make_config(9, 400, 100, tau0=3 * u * D / Re, u0=u, precision=precision, body=L.BODY_SPHERE,
                       center=(80.0, 50.0, 0.0), r2=(D / 2) ** 2, seed=SEED, perturb=1e-3,
                       **{"force_interval": 10, **kw})
-    return Workload("cfg0_d2q9_cylinder_400x100_re100", c, 20000, D, D, u)
+    return Workload("d2q9_cylinder_400x100_re100", c, 20000, D, D, u)
 
 
-def cfg1_sphere3d(precision=L.FP32, **kw) -> Workload:
+def sphere3d_256x128x128(precision=L.FP32, **kw) -> Workload:
     """D3Q27 256x128x128, sphere D=24 at (64,64,64), Re=200, u=0.05, 10 000 steps."""
     u, D, Re = 0.05, 24.0, 200.0
     c = L.make_config(27, 256, 128, 128, tau0=3 * u * D / Re, u0=u, precision=precision, body=L.BODY_SPHERE,
                       center=(64.0, 64.0, 64.0), r2=(D / 2) ** 2, seed=SEED, perturb=1e-3,
                       **{"force_interval": 10, **kw})
-    return Workload("cfg1_d3q27_sphere_256x128x128_re200", c, 10000, D, math.pi * D * D / 4, u)
+    return Workload("d3q27_sphere_256x128x128_re200", c, 10000, D, math.pi * D * D / 4, u)
 
 
-def cfg2_zcyl3d(precision=L.FP16S, **kw) -> Workload:
+def zcylinder3d_512x256x256(precision=L.FP16S, **kw) -> Workload:
     """D3Q27 512x256x256, z-spanning cylinder D=40 at (128,128), Re=300, u=0.05, 20 000 steps."""
     u, D, Re = 0.05, 40.0, 300.0
     c = L.make_config(27, 512, 256, 256, tau0=3 * u * D / Re, u0=u, precision=precision, body=L.BODY_ZCYL,
                       center=(128.0, 128.0, 0.0), r2=(D / 2) ** 2, seed=SEED, perturb=1e-3,
                       **{"force_interval": 10, **kw})
-    return Workload("cfg2_d3q27_zcylinder_512x256x256_re300", c, 20000, D, D * 256, u)
+    return Workload("d3q27_zcylinder_512x256x256_re300", c, 20000, D, D * 256, u)
 
 
 def smoothed_disk(nx, ny, cx, cy, R, width=1.0, noise=0.01, seed=SEED):
@@ -79,13 +79,13 @@ def smoothed_ball(nx, ny, nz, c, R, width=1.0, noise=0.01, seed=SEED):
     return np.clip(s, 0.0, 1.0)
 
 
-def cfg3_adjoint2d(**kw) -> Workload:
-    """D2Q9 adjoint on the cfg0 geometry: solid fraction = smoothed disk + 0.01 N(0,1); J = mean Cd
+def adjoint2d_400x100(**kw) -> Workload:
+    """D2Q9 adjoint on the 400x100 cylinder geometry: solid fraction = smoothed disk + 0.01 N(0,1); J = mean Cd
     over steps 4000-8000; checkpoint every 500; FP64."""
     u, D, Re = 0.05, 20.0, 100.0
     c = L.make_config(9, 400, 100, tau0=3 * u * D / Re, u0=u, precision=L.FP64, body=L.BODY_NONE,
                       seed=SEED, perturb=1e-3, **kw)
-    w = Workload("cfg3_d2q9_adjoint_400x100_re100", c, 8000, D, D, u, window=(4000, 8000), checkpoint=500)
+    w = Workload("d2q9_adjoint_400x100_re100", c, 8000, D, D, u, window=(4000, 8000), checkpoint=500)
     w.extra["sfrac"] = smoothed_disk(400, 100, 80.0, 50.0, D / 2)
     return w
 
@@ -98,7 +98,7 @@ def legendre_coefficients(seed=SEED, n=8, amplitude=0.05):
     return amplitude * a / np.abs(a).sum()
 
 
-def cfg4_adjoint3d(precision=L.FP32, Re=100.0, **kw) -> Workload:
+def adjoint3d_192x96x96(precision=L.FP32, Re=100.0, **kw) -> Workload:
     """D3Q27 adjoint 192x96x96, sphere D=20 as a solid-fraction field, parabola x Legendre inlet
     (8 seeded coefficients, sum |a_k| <= 0.05); J = mean Cd over steps 2000-6000; checkpoint every
     250; FP32 with FP64 reductions.  Re is not fixed by the workload description: Re = 100 on
@@ -107,15 +107,15 @@ def cfg4_adjoint3d(precision=L.FP32, Re=100.0, **kw) -> Workload:
     coef = tuple(legendre_coefficients())
     c = L.make_config(27, 192, 96, 96, tau0=3 * u * D / Re, u0=u, precision=precision, body=L.BODY_NONE,
                       inlet=L.INLET_LEGENDRE, coef=coef, seed=SEED, perturb=1e-3, **kw)
-    w = Workload("cfg4_d3q27_adjoint_192x96x96", c, 6000, D, math.pi * D * D / 4, u, window=(2000, 6000),
+    w = Workload("d3q27_adjoint_192x96x96", c, 6000, D, math.pi * D * D / 4, u, window=(2000, 6000),
                  checkpoint=250)
     w.extra["sfrac"] = smoothed_ball(192, 96, 96, (48.0, 48.0, 48.0), D / 2)
     w.extra["coef"] = np.array(coef)
     return w
 
 
-ALL = {"cfg0": cfg0_cylinder2d, "cfg1": cfg1_sphere3d, "cfg2": cfg2_zcyl3d, "cfg3": cfg3_adjoint2d,
-       "cfg4": cfg4_adjoint3d}
+ALL = {f.__name__: f for f in (cylinder2d_400x100, sphere3d_256x128x128, zcylinder3d_512x256x256, adjoint2d_400x100,
+                                 adjoint3d_192x96x96)}
 
 
 def strouhal(cl: np.ndarray, dt_steps: float, D: float, u: float, min_periods: int = 3,

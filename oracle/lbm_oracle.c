@@ -2,7 +2,7 @@
  * lbm_oracle.c -- CPU FP64 restatement of the reference LBM hot path (see lbm_oracle.h).
  *
  * TEST INFRASTRUCTURE ONLY: loaded by tests/, the smoke check and bench.py's cpu_baseline
- * leg as the checker.  Never linked into the product library.
+ * leg, which compare the product against it.  Never linked into the product library.
  *
  * Built with -ffp-contract=off so every expression rounds exactly as written; the
  * reference's pow(x, 2) is written x*x (both compilers fold pow(x, 2.0) to x*x).

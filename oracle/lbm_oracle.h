@@ -1,9 +1,9 @@
 /*
  * lbm_oracle.h -- CPU FP64 restatement of the reference lattice Boltzmann hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, the smoke check and
- * the cpu_baseline leg of bench.py may load it; the product (the HIP library under
- * cylinder-flow_amd/) never links, loads or calls it.
+ * TEST INFRASTRUCTURE ONLY.  tests/, the smoke check and the cpu_baseline leg of bench.py
+ * may load this library to compare the product against it; the product (the HIP library
+ * under cylinder-flow_amd/) never links, loads or calls it.
  *
  * Two boundary modes are restated:
  *

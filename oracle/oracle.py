@@ -1,7 +1,8 @@
 """ctypes binding of the CPU FP64 oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` as the checker.  The product library never loads it.
+``cpu_baseline`` leg of ``bench.py``, which compare the product against it.  The product
+library never loads it.
 """
 from __future__ import annotations
 

@@ -72,17 +72,17 @@ def test_torch_wrapper_chains_the_adjoint(pkg):
     assert abs(s.grad[i].item() - fd) <= 1e-6 * max(abs(fd), 1e-4)
 
 
-def test_cfg3_adjoint_matches_finite_differences(pkg):
+def test_adjoint2d_matches_finite_differences(pkg):
     L = pkg.lbm
-    W = pkg.configs.cfg3_adjoint2d()
+    W = pkg.configs.adjoint2d_400x100()
     n0, n1 = W.window
     sf = W.extra["sfrac"]
-    print("[adjoint] cfg3 400x100 FP64, 8000 steps, K=500", flush=True)
+    print("[adjoint] 2D cylinder 400x100 FP64, 8000 steps, K=500", flush=True)
     with L.Solver(W.cfg) as s:
         s.set_solid_fraction(sf)
         s.initialize()
         J, ds, du, tf, ta = s.adjoint(n0, n1, W.cd_scale(), W.checkpoint)
-    print(f"[adjoint] cfg3 J = mean Cd = {J:.6f}; forward {tf:.1f} ms, adjoint {ta:.1f} ms ({ta / tf:.2f}x)")
+    print(f"[adjoint] 2D cylinder J = mean Cd = {J:.6f}; forward {tf:.1f} ms, adjoint {ta:.1f} ms ({ta / tf:.2f}x)")
     rng = np.random.default_rng(W.cfg.seed)
     eps = 1e-6
     # 32 seeded nodes where s +- eps stays inside [0, 1] (the clamp leaves s exactly 0 or 1
@@ -99,7 +99,7 @@ def test_cfg3_adjoint_matches_finite_differences(pkg):
         fd = (objective(pkg, W.cfg, n0, n1, W.cd_scale(), sfrac=sp) -
               objective(pkg, W.cfg, n0, n1, W.cd_scale(), sfrac=sm)) / (2 * eps)
         errs.append(abs(ds[0, y, x] - fd) / max(abs(fd), 1e-3 * np.abs(ds).max()))
-    print(f"[adjoint] cfg3 dJ/ds vs FD on 32 nodes: max rel err {max(errs):.2e}")
+    print(f"[adjoint] 2D cylinder dJ/ds vs FD on 32 nodes: max rel err {max(errs):.2e}")
     assert max(errs) <= 1e-4
     # uniform inlet velocity: dJ/dU = sum over inlet nodes of dJ/du_in
     c = L.LbmConfig.from_buffer_copy(W.cfg)
@@ -108,14 +108,14 @@ def test_cfg3_adjoint_matches_finite_differences(pkg):
         c.u0 = W.cfg.u0 + sgn * eps
         vals.append(objective(pkg, c, n0, n1, W.cd_scale(), sfrac=sf))
     fd = (vals[0] - vals[1]) / (2 * eps)
-    print(f"[adjoint] cfg3 dJ/du_in: adjoint {du.sum():.8e} FD {fd:.8e}")
+    print(f"[adjoint] 2D cylinder dJ/du_in: adjoint {du.sum():.8e} FD {fd:.8e}")
     assert abs(du.sum() - fd) <= 1e-4 * abs(fd)
 
 
-def test_cfg4_adjoint_coefficients_and_cost(pkg):
+def test_adjoint3d_coefficients_and_cost(pkg):
     from cylinder_flow_amd.autograd import legendre_basis
     L = pkg.lbm
-    W = pkg.configs.cfg4_adjoint3d()
+    W = pkg.configs.adjoint3d_192x96x96()
     n0, n1 = W.window
     sf = W.extra["sfrac"]
     phi = legendre_basis(96, 96).numpy()
@@ -123,13 +123,13 @@ def test_cfg4_adjoint_coefficients_and_cost(pkg):
     for prec in ("fp32", "fp64"):
         c = L.LbmConfig.from_buffer_copy(W.cfg)
         c.precision = L.PRECISIONS[prec]
-        print(f"[adjoint] cfg4 192x96x96 {prec}, 6000 steps, K=250", flush=True)
+        print(f"[adjoint] 3D sphere 192x96x96 {prec}, 6000 steps, K=250", flush=True)
         with L.Solver(c) as s:
             s.set_solid_fraction(sf)
             s.initialize()
             J, ds, du, tf, ta = s.adjoint(n0, n1, W.cd_scale(), W.checkpoint)
         grads[prec] = (J, ds, np.einsum("kzy,zy->k", phi, du), tf, ta)
-        print(f"[adjoint] cfg4 {prec}: J {J:.6f}, forward {tf:.0f} ms, adjoint {ta:.0f} ms, ratio {ta / tf:.2f}")
+        print(f"[adjoint] 3D sphere {prec}: J {J:.6f}, forward {tf:.0f} ms, adjoint {ta:.0f} ms, ratio {ta / tf:.2f}")
     J32, ds32, da32, tf32, ta32 = grads["fp32"]
     J64, ds64, da64, tf64, ta64 = grads["fp64"]
     assert ta32 / tf32 <= 3.0
@@ -148,14 +148,14 @@ def test_cfg4_adjoint_coefficients_and_cost(pkg):
         fd[k] = (vals[0] - vals[1]) / (2 * eps)
     err64 = np.abs(da64 - fd) / np.abs(fd).max()
     err32 = np.abs(da32 - fd) / np.abs(fd).max()
-    print(f"[adjoint] cfg4 dJ/da FD {fd}\n  fp64 adjoint err {err64.max():.2e}, fp32 adjoint err {err32.max():.2e}, "
+    print(f"[adjoint] 3D sphere dJ/da FD {fd}\n  fp64 adjoint err {err64.max():.2e}, fp32 adjoint err {err32.max():.2e}, "
           f"fp32 vs fp64 dJ/ds rel-L2 {rel_l2(ds32, ds64):.2e}")
     assert err64.max() <= 1e-4
     assert err32.max() <= 5e-2
     assert rel_l2(ds32, ds64) <= 5e-2
 
 
-def test_reduced_cfg4_matches_cpu_fp64_adjoint(pkg, orc):
+def test_reduced_adjoint3d_matches_cpu_fp64_adjoint(pkg, orc):
     from cylinder_flow_amd.autograd import legendre_basis
     L = pkg.lbm
     nx, ny, nz, D = 48, 24, 24, 6.0
@@ -166,7 +166,7 @@ def test_reduced_cfg4_matches_cpu_fp64_adjoint(pkg, orc):
     scale = 2.0 / (0.05 ** 2 * np.pi * D * D / 4)
     o = orc.make_cfg(27, nx, ny, nz, tau0=3 * 0.05 * D / 40.0, u0=0.05, inlet=orc.INLET_LEGENDRE, coef=tuple(coef),
                      perturb=1e-3)
-    print("[adjoint] reduced cfg4 CPU FP64 reference", flush=True)
+    print("[adjoint] reduced 3D sphere CPU FP64 reference", flush=True)
     Jo, dso, duo = orc.adjoint(o, sfrac=sf, n0=n0, n1=n1, scale=scale, K=K)
     phi = legendre_basis(ny, nz).numpy()
     for prec, tol in (("fp64", 1e-9), ("fp32", 5e-3)):
@@ -177,5 +177,5 @@ def test_reduced_cfg4_matches_cpu_fp64_adjoint(pkg, orc):
             s.initialize()
             J, ds, du, tf, ta = s.adjoint(n0, n1, scale, K)
         e = (abs(J - Jo) / abs(Jo), rel_l2(ds, dso), rel_l2(np.einsum("kzy,zy->k", phi, du), np.einsum("kzy,zy->k", phi, duo)))
-        print(f"[adjoint] reduced cfg4 {prec} vs CPU FP64: J {e[0]:.2e} dJ/ds {e[1]:.2e} dJ/da {e[2]:.2e}")
+        print(f"[adjoint] reduced 3D sphere {prec} vs CPU FP64: J {e[0]:.2e} dJ/ds {e[1]:.2e} dJ/da {e[2]:.2e}")
         assert max(e) <= tol
