@@ -46,9 +46,9 @@ PLAIN_CASES = {
 }
 
 
-def run_pair(pkg, orc, name, precision, sfrac=False):
+def run_pair(pkg, orc, name, precision, sfrac=False, store_uref=0.0):
     Q, nx, ny, nz, kw, steps = PLAIN_CASES[name]
-    g, o = plain_pair(pkg, orc, Q, nx, ny, nz, precision=precision, **kw)
+    g, o = plain_pair(pkg, orc, Q, nx, ny, nz, precision=precision, store_uref=store_uref, **kw)
     s = None
     if sfrac:
         shape = (nz, ny, nx)
@@ -92,10 +92,14 @@ def test_plain_fp32_within_budget(pkg, orc, name):
     assert rel_l2(gs.populations(), os_.populations()) <= 2e-5
 
 
+@pytest.mark.parametrize("store_uref", [0.0, 0.05])
 @pytest.mark.parametrize("precision,budget", [("fp16s", 3e-3), ("bf16s", 3e-2)])
 @pytest.mark.parametrize("name", ["d2q9_disk_walls_uniform", "d3q27_sphere_walls_parabolic"])
-def test_plain_16bit_within_budget(pkg, orc, name, precision, budget):
-    gs, os_ = run_pair(pkg, orc, name, pkg.lbm.PRECISIONS[precision])
+def test_plain_16bit_within_budget(pkg, orc, name, precision, budget, store_uref):
+    """16-bit storage against the FP64 oracle; store_uref = u0 stores deviations from the free
+    stream's equilibrium (opposite directions then have different offsets, so the nodes next
+    to the body take the per-node path instead of the bounce-back copies)."""
+    gs, os_ = run_pair(pkg, orc, name, pkg.lbm.PRECISIONS[precision], store_uref=store_uref)
     u_g = gs.fields()[1]
     u_o = os_.fields()[1]
     err = rel_l2(u_g, u_o)
@@ -229,6 +233,19 @@ def test_adjoint_matches_cpu_adjoint(pkg, orc, Q, shape, prec, tol):
     (27, (3, 3, 3), dict()),
 ])
 def test_fast_path_matches_per_node_path(pkg, orc, monkeypatch, prec, V, Q, shape, kw):
+    _fast_vs_per_node(pkg, monkeypatch, prec, V, Q, shape, kw, 0.0)
+
+
+@pytest.mark.parametrize("prec,V", [("fp16s", 2), ("fp16s", 4), ("bf16s", 2)])
+@pytest.mark.parametrize("Q,shape,kw", [
+    (27, (14, 20, 64), dict(body=1, center=(20.0, 9.5, 6.5), r2=9.0, inlet=1)),
+    (9, (1, 16, 128), dict(body=3, mask="edges")),
+])
+def test_fast_path_matches_per_node_path_with_storage_offsets(pkg, monkeypatch, prec, V, Q, shape, kw):
+    _fast_vs_per_node(pkg, monkeypatch, prec, V, Q, shape, kw, 0.05)
+
+
+def _fast_vs_per_node(pkg, monkeypatch, prec, V, Q, shape, kw, store_uref):
     """The packed path (V nodes per lane) followed by the per-node recomputation of the nodes
     it gets wrong, and the per-node path on every fluid node, give bit-identical populations,
     fields and forces.  The "edges" mask puts solid nodes on the inlet and outlet lanes,
@@ -248,12 +265,12 @@ def test_fast_path_matches_per_node_path(pkg, orc, monkeypatch, prec, V, Q, shap
         mask[:, ny // 2, 75] = 1             # isolated node, odd x
         mask[:, ny // 2 + 1, 100:104] = 1
     g = L.make_config(Q, nx, ny, nz, tau0=0.05, u0=0.05, precision=L.PRECISIONS[prec], bc_mode=L.BC_PLAIN,
-                      perturb=1e-3, force_interval=3, **kw)
+                      perturb=1e-3, force_interval=3, store_uref=store_uref, **kw)
     monkeypatch.setenv("LBM_NODES_PER_LANE", str(V))
     out = []
     for slow in ("0", "1"):
         monkeypatch.setenv("LBM_FORCE_SLOW_PATH", slow)
-        print(f"[parity] fast/per-node {prec} V={V} Q={Q} {shape} slow={slow}", flush=True)
+        print(f"[parity] fast/per-node {prec} V={V} Q={Q} {shape} store_uref={store_uref} slow={slow}", flush=True)
         s = pkg.Solver(g)
         if mask is not None:
             s.set_mask(mask)
