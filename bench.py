@@ -11,10 +11,12 @@ A "step" is one fused lattice update of the whole domain: the bounce-back copies
 slots (k_link_pairs, or k_link_v with the momentum exchange on force steps), k_fast_v (packed
 pull-collide-store over every node) and, when its list is not empty, k_fix_v, back to back on
 the solver's stream.  MLUPS = nodes * steps / seconds over
-exactly --steps steps bracketed by stream synchronisation.  The roofline uses the algorithmic
-2*Q*sizeof(storage) bytes per node update and the step's average time measured with HIP events
-on the solver's own stream (a second context with forces off, so only the step's kernels run in
-the timed region).
+exactly --steps steps bracketed by stream synchronisation.  The roofline is that of the
+dominant kernel, k_fast_v: the algorithmic 2*Q*sizeof(storage) bytes per node update times the
+nodes one launch updates, over its average launch duration measured with HIP events recorded
+just before and after each launch on the solver's own stream (a second context with forces
+off).  roofline.step gives the same for the whole step (events around the steps), and
+roofline.traffic the counter-measured HBM bytes per launch (profiles/traffic.json).
 """
 from __future__ import annotations
 
@@ -60,15 +62,20 @@ def run_leg(pkg, precision, steps, warmup, force_interval):
 
 
 def kernel_roofline(pkg, precision, steps, bytes_per_update, nodes):
-    """Average duration of one step's kernels alone (forces off), HIP events on their stream."""
+    """Forces off, HIP events on the solver's stream: the average duration of the dominant
+    kernel (k_fast_v, events recorded just before and after each launch) and of the whole step
+    (k_link_pairs + k_fast_v, events around the steps, in a run without the per-kernel events).
+    Returns (kernel seconds, kernel GB/s, step seconds, step GB/s) per launch / step."""
     W = pkg.configs.zcylinder3d_512x256x256(precision=precision, force_interval=0)
     s = pkg.Solver(W.cfg).initialize()
     s.step(5)
-    ms = s.step_timed(steps)
+    ms_step = s.step_timed(steps)
+    _, ms_kernel = s.step_kernel_timed(steps)
     s.close()
-    per_launch = ms / steps / 1e3
-    achieved = bytes_per_update * nodes / per_launch / 1e9
-    return per_launch, achieved
+    per_launch = ms_kernel / steps / 1e3
+    per_step = ms_step / steps / 1e3
+    algo = bytes_per_update * nodes
+    return per_launch, algo / per_launch / 1e9, per_step, algo / per_step / 1e9
 
 
 def traffic_from_profiles(name):
@@ -283,7 +290,7 @@ def main():
     s16.close()
     mlups16 = nodes * K / wall16 / 1e6
     log(f"fp16s {mlups16:.0f} MLUPS, {wall16 / K * 1e3:.4f} ms/step; kernel roofline run")
-    t16, ach16 = kernel_roofline(pkg, L.FP16S, K, info16.bytes_per_update, nodes)
+    t16, ach16, st16, sach16 = kernel_roofline(pkg, L.FP16S, K, info16.bytes_per_update, nodes)
     out = {
         "metric": METRIC, "value": round(mlups16, 1), "unit": "MLUPS", "n_gpus": 1, "steps": K, "warmup": Wm,
         "ms_per_step": round(wall16 / K * 1e3, 5), "higher_is_better": True, "scaling": "weak",
@@ -292,23 +299,29 @@ def main():
                    "storage": "fp16 shifted (f_i - w_i rho0)/w_i", "collision": "fp32",
                    "force_interval": 10, "body": "z-cylinder D=40", "Re": 300},
         "roofline": {"bound": "hbm", "achieved": round(ach16, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach16 / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles("fp16s"),
-                     "kernel": f"step: k_link_pairs + k_fast_v<27,SF16,{info16.nodes_per_lane}>", "kernel_ms": round(t16 * 1e3, 5),
-                     "bytes_per_update": info16.bytes_per_update},
+                     "frac": round(ach16 / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles("fp16s_k_fast_v"),
+                     "kernel": f"k_fast_v<27,SF16,{info16.nodes_per_lane}>", "kernel_ms": round(t16 * 1e3, 5),
+                     "bytes_per_update": info16.bytes_per_update,
+                     "step": {"kernels": "k_link_pairs + k_fast_v", "ms": round(st16 * 1e3, 5),
+                              "achieved": round(sach16, 1), "frac": round(sach16 / HBM_PEAK_GBS, 4),
+                              "traffic": traffic_from_profiles("fp16s")}},
         "forces_finite": bool(forces.size and (abs(forces[:, 1:]) < 1e30).all()),
     }
     if not a.no_fp32:
         log("fp32 leg")
         s32, W32, wall32, info32 = run_leg(pkg, L.FP32, K, Wm, 10)
         s32.close()
-        t32, ach32 = kernel_roofline(pkg, L.FP32, K, info32.bytes_per_update, nodes)
+        t32, ach32, st32, sach32 = kernel_roofline(pkg, L.FP32, K, info32.bytes_per_update, nodes)
         out["fp32"] = {"value": round(nodes * K / wall32 / 1e6, 1), "unit": "MLUPS",
                        "ms_per_step": round(wall32 / K * 1e3, 5),
                        "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS,
                                     "unit": "GB/s", "frac": round(ach32 / HBM_PEAK_GBS, 4),
-                                    "traffic": traffic_from_profiles("fp32"),
-                                    "kernel": f"step: k_link_pairs + k_fast_v<27,SF32,{info32.nodes_per_lane}>",
-                                    "kernel_ms": round(t32 * 1e3, 5), "bytes_per_update": info32.bytes_per_update}}
+                                    "traffic": traffic_from_profiles("fp32_k_fast_v"),
+                                    "kernel": f"k_fast_v<27,SF32,{info32.nodes_per_lane}>",
+                                    "kernel_ms": round(t32 * 1e3, 5), "bytes_per_update": info32.bytes_per_update,
+                                    "step": {"kernels": "k_link_pairs + k_fast_v", "ms": round(st32 * 1e3, 5),
+                                             "achieved": round(sach32, 1), "frac": round(sach32 / HBM_PEAK_GBS, 4),
+                                             "traffic": traffic_from_profiles("fp32")}}}
     if not a.no_adjoint:
         log("adjoint leg")
         out["adjoint"] = adjoint_ratio(pkg)

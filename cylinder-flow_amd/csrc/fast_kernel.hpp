@@ -442,13 +442,16 @@ static hipError_t fast_launch(const StepBufs &b, const FusedArgs &a, hipStream_t
     }
     // the aligned pull needs the whole row in one block
     const unsigned lds = b.fast_lds;
+    if (b.ev_bulk[0] && (e = hipEventRecord(b.ev_bulk[0], s)) != hipSuccess) return e;
     if (sizeof(T) == 2 && V == 2 && sh.gx == 1 && b.aligned_pull)
         hipLaunchKernelGGL((k_fast_v<Q, S, V, MACROS, (sizeof(T) == 2 && V == 2)>), dim3(sh.gx, a.g.ny, a.g.nz),
                            dim3(sh.bx), lds, s, (const T *)b.A, (T *)b.B, b.uin, b.macros, a, tab);
     else
         hipLaunchKernelGGL((k_fast_v<Q, S, V, MACROS, false>), dim3(sh.gx, a.g.ny, a.g.nz), dim3(sh.bx), lds, s,
                            (const T *)b.A, (T *)b.B, b.uin, b.macros, a, tab);
-    if ((e = hipGetLastError()) != hipSuccess || b.nfix == 0) return e;
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (b.ev_bulk[1] && (e = hipEventRecord(b.ev_bulk[1], s)) != hipSuccess) return e;
+    if (b.nfix == 0) return hipSuccess;
     const dim3 grid((unsigned)((b.nfix + 255) / 256));
     if (b.sfrac)
         hipLaunchKernelGGL((k_fix_v<Q, S, FORCE, MACROS, true>), grid, dim3(256), 0, s, (const T *)b.A, (T *)b.B, b.fix,

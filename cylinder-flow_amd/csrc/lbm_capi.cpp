@@ -645,10 +645,12 @@ StepBufs step_bufs(const lbm_ctx *c, const void *A, void *B) {
     b.macros = c->macros;
     b.aligned_pull = c->aligned_pull;
     b.fast_lds = c->fast_lds;
+    b.ev_bulk[0] = b.ev_bulk[1] = nullptr;
     return b;
 }
 
-static int step_n(lbm_ctx *c, int n) {
+// evs: null, or 2n events recorded around each step's bulk kernel (plain mode)
+static int step_n(lbm_ctx *c, int n, hipEvent_t *evs = nullptr) {
     const int fi = c->cfg.force_interval;
     const int nforce = fi > 0 ? (int)((c->step + n) / fi - c->step / fi) : 0;
     int rc = grow_hist(c, c->nrec + nforce + 1);
@@ -670,7 +672,11 @@ static int step_n(lbm_ctx *c, int n) {
     for (int k = 0; k < n; k++) {
         c->step++;
         const bool force = fi > 0 && c->step % fi == 0;
-        const StepBufs b = step_bufs(c, c->A, c->B);
+        StepBufs b = step_bufs(c, c->A, c->B);
+        if (evs) {
+            b.ev_bulk[0] = evs[2 * k];
+            b.ev_bulk[1] = evs[2 * k + 1];
+        }
         HIP_TRY(c, plain_step(c->Q, c->prec, c->V, b, force, k == n - 1, a, c->stream));
         if (force) HIP_TRY(c, force_reduce(c->partials, c->nslots, c->partials + 3 * (size_t)c->nslots, c->hist, c->nrec++, c->step, c->stream));
         std::swap(c->A, c->B);
@@ -702,6 +708,44 @@ extern "C" int lbm_step_timed(lbm_ctx *c, int n, double *ms) {
     (void)hipEventDestroy(e1);
     *ms = t;
     return rc;
+}
+
+extern "C" int lbm_step_kernel_timed(lbm_ctx *c, int n, double *ms_steps, double *ms_bulk) {
+    if (!c || !ms_steps || !ms_bulk) return LBM_ERR_INVALID;
+    if (!c->initialized) return set_err(c, LBM_ERR_STATE, "lbm_step_kernel_timed before lbm_initialize");
+    if (c->cfg.bc_mode != LBM_BC_PLAIN) return set_err(c, LBM_ERR_INVALID, "lbm_step_kernel_timed times the plain mode");
+    if (n <= 0 || n > 100000) return set_err(c, LBM_ERR_INVALID, "lbm_step_kernel_timed: n must be in [1, 100000]");
+    std::vector<hipEvent_t> ev(2 * (size_t)n + 2, nullptr);
+    auto release = [&]() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    };
+    for (hipEvent_t &e : ev) {
+        const hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) {
+            e = nullptr;
+            release();
+            return set_err(c, LBM_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(r));
+        }
+    }
+    hipError_t r = hipEventRecord(ev[2 * n], c->stream);
+    int rc = r == hipSuccess ? step_n(c, n, ev.data()) : LBM_ERR_HIP;
+    if (rc == LBM_OK) r = hipEventRecord(ev[2 * n + 1], c->stream);
+    if (rc == LBM_OK && r == hipSuccess) r = hipEventSynchronize(ev[2 * n + 1]);
+    float t = 0.f;
+    double bulk = 0.0;
+    if (rc == LBM_OK && r == hipSuccess) r = hipEventElapsedTime(&t, ev[2 * n], ev[2 * n + 1]);
+    for (int k = 0; k < n && rc == LBM_OK && r == hipSuccess; k++) {
+        float tk = 0.f;
+        r = hipEventElapsedTime(&tk, ev[2 * k], ev[2 * k + 1]);
+        bulk += tk;
+    }
+    release();
+    if (rc != LBM_OK) return rc;
+    if (r != hipSuccess) return set_err(c, LBM_ERR_HIP, std::string("lbm_step_kernel_timed: ") + hipGetErrorString(r));
+    *ms_steps = t;
+    *ms_bulk = bulk;
+    return LBM_OK;
 }
 
 extern "C" int lbm_sync(lbm_ctx *c) {

@@ -298,6 +298,10 @@ template <int Q, bool FORCE, bool MACROS>
 static hipError_t step_f64(const StepBufs &b, const FusedArgs &a, hipStream_t s) {
     using S = SF64;
     const Shape sh = lane_shape(a.g, 1);
+    if (b.ev_bulk[0]) {
+        const hipError_t e = hipEventRecord(b.ev_bulk[0], s);
+        if (e != hipSuccess) return e;
+    }
     if (b.sfrac)
         hipLaunchKernelGGL((k_fused<Q, S, FORCE, MACROS, true>), grid_of(a.g, sh), dim3(sh.bx), 0, s,
                            (const double *)b.A, (double *)b.B, b.flags, (const double *)b.sfrac, b.uin, b.block_slot,
@@ -306,7 +310,9 @@ static hipError_t step_f64(const StepBufs &b, const FusedArgs &a, hipStream_t s)
         hipLaunchKernelGGL((k_fused<Q, S, FORCE, MACROS, false>), grid_of(a.g, sh), dim3(sh.bx), 0, s,
                            (const double *)b.A, (double *)b.B, b.flags, (const double *)nullptr, b.uin, b.block_slot,
                            b.partials, b.macros, a);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && b.ev_bulk[1]) e = hipEventRecord(b.ev_bulk[1], s);
+    return e;
 }
 
 template <int Q>

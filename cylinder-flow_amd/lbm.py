@@ -81,6 +81,7 @@ def lib():
             "lbm_get_forces": (I, [P, P, I, C.POINTER(I)]),
             "lbm_info_get": (I, [P, C.POINTER(LbmInfo)]),
             "lbm_snapshot_async": (I, [P, C.c_char_p, I]),
+            "lbm_step_kernel_timed": (I, [P, I, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
             "lbm_snapshot_wait": (I, [P]),
             "lbm_adjoint": (I, [P, I, I, C.c_double, I, P, P, P, P, P]),
         }
@@ -202,6 +203,12 @@ class Solver:
         ms = C.c_double()
         self._chk(lib().lbm_step_timed(self.h, n, C.byref(ms)))
         return ms.value
+
+    def step_kernel_timed(self, n):
+        """(ms of the n steps, ms summed over the steps of the bulk kernel alone)"""
+        t, k = C.c_double(), C.c_double()
+        self._chk(lib().lbm_step_kernel_timed(self.h, n, C.byref(t), C.byref(k)))
+        return t.value, k.value
 
     def sync(self):
         self._chk(lib().lbm_sync(self.h))

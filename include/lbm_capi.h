@@ -123,6 +123,11 @@ int lbm_initialize(lbm_ctx *ctx);
 int lbm_step(lbm_ctx *ctx, int n);
 /* the same, bracketed by HIP events on the context stream; *ms = elapsed milliseconds */
 int lbm_step_timed(lbm_ctx *ctx, int n, double *ms);
+/* n steps as lbm_step, timed with HIP events on the context stream: *ms_steps = the n steps,
+ * *ms_bulk = the sum over the steps of the bulk kernel alone (k_fast_v, or k_fused for FP64
+ * storage), between events recorded just before and after its launch.  Plain mode only,
+ * 1 <= n <= 100000.  A measurement entry point with no counterpart in the reference. */
+int lbm_step_kernel_timed(lbm_ctx *ctx, int n, double *ms_steps, double *ms_bulk);
 /* drains the context stream and checks the forces recorded since the last check */
 int lbm_sync(lbm_ctx *ctx);
 

@@ -308,3 +308,24 @@ def test_launch_variants_give_identical_results(pkg, monkeypatch, prec, env, val
             out.append((s.populations(), s.forces()))
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16s"])
+def test_kernel_timed_steps_equal_plain_steps(pkg, prec):
+    """lbm_step_kernel_timed (events around each step's bulk kernel) steps the same state as
+    lbm_step, and the bulk kernel's summed time lies inside the steps' time."""
+    L = pkg.lbm
+    g = L.make_config(27, 64, 20, 16, tau0=0.05, u0=0.05, precision=L.PRECISIONS[prec], body=L.BODY_SPHERE,
+                      center=(16.0, 9.5, 7.5), r2=9.0, perturb=1e-3, force_interval=4)
+    out = []
+    for timed in (False, True):
+        with L.Solver(g) as s:
+            s.initialize()
+            if timed:
+                t, k = s.step_kernel_timed(12)
+                print(f"[parity] {prec} 12 steps {t:.3f} ms, bulk kernel {k:.3f} ms", flush=True)
+                assert 0.0 < k < t
+            else:
+                s.step(12)
+            out.append((s.populations(), s.forces()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
