@@ -89,7 +89,8 @@ struct RefGeo {
     int Q, D, g;                 // ghost width 3 (3D) / 2 (2D)
     int glx, gly, glz;           // interior extents
     int lx, ly, lz;              // padded extents
-    size_t plane;                // lx*ly*lz
+    int px;                      // row pitch in elements (>= lx; 3D rows start 256-byte aligned)
+    size_t plane;                // px*ly*lz
     double tau0, u0;
     int variant;                 // D2Q9: 0 the MPI variant, 1 the single-node variant
 };

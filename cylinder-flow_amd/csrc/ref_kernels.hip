@@ -57,8 +57,9 @@ __global__ void __launch_bounds__(256) k_ref_init(RefGeo r, RefBody b, double *_
     using L = Lat<Q>;
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
     if (i >= r.lx) return;
-    const size_t n = (size_t)i + (size_t)r.lx * ((size_t)j + (size_t)r.ly * k);
-    const int ran = ran_in ? ran_in[n] : (int)(ref_splitmix64(b.seed ^ ref_splitmix64(n)) % 100ull);
+    const size_t n = (size_t)i + (size_t)r.px * ((size_t)j + (size_t)r.ly * k);
+    const size_t nl = (size_t)i + (size_t)r.lx * ((size_t)j + (size_t)r.ly * k);   // unpadded: ran's index
+    const int ran = ran_in ? ran_in[nl] : (int)(ref_splitmix64(b.seed ^ ref_splitmix64(nl)) % 100ull);
     const int x = i - r.g, y = j - r.g, z = (L::D == 3) ? k - r.g : 0;
     int inside = 0;
     if (b.kind == LBM_BODY_SPHERE || b.kind == LBM_BODY_ZCYL) {
@@ -86,7 +87,7 @@ __global__ void __launch_bounds__(256) k_ref_init(RefGeo r, RefBody b, double *_
     });
 }
 
-#define RIDX(r, ii, i, j, k) ((size_t)(ii) * (r).plane + (size_t)((long)(i) + (long)(r).lx * ((long)(j) + (long)(r).ly * (long)(k))))
+#define RIDX(r, ii, i, j, k) ((size_t)(ii) * (r).plane + (size_t)((long)(i) + (long)(r).px * ((long)(j) + (long)(r).ly * (long)(k))))
 
 // 3D fills, one kernel per face: 3D MPI+KOKKOS+MPIIO/lbm.cpp:278-347.
 // face: 0 front (y lo), 1 back (y hi), 2 bottom (z lo), 3 top (z hi), 4 inlet, 5 outlet
@@ -161,13 +162,13 @@ __global__ void __launch_bounds__(256) k_ref_usrbc(RefGeo r, double *__restrict_
     const int i = blockIdx.x * blockDim.x + threadIdx.x + r.g, j = blockIdx.y + r.g;
     const int k = blockIdx.z + (L::D == 3 ? r.g : 0);
     if (i >= r.lx - r.g) return;
-    const long n = (long)i + (long)r.lx * ((long)j + (long)r.ly * k);
+    const long n = (long)i + (long)r.px * ((long)j + (long)r.ly * k);
     if (usr[n] != 0) return;
     static_for<0, Q>([&](auto I) {
         constexpr int ii = I;
-        const long nb = n + L::ex(ii) + (long)r.lx * (L::ey(ii) + (long)r.ly * L::ez(ii));
+        const long nb = n + L::ex(ii) + (long)r.px * (L::ey(ii) + (long)r.ly * L::ez(ii));
         if (usr[nb] == 1) {
-            const long n2 = n + 2 * (L::ex(ii) + (long)r.lx * (L::ey(ii) + (long)r.ly * L::ez(ii)));
+            const long n2 = n + 2 * (L::ex(ii) + (long)r.px * (L::ey(ii) + (long)r.ly * L::ez(ii)));
             f[(size_t)ii * r.plane + n] = f[(size_t)L::opp(ii) * r.plane + n2];
         }
     });
@@ -183,11 +184,11 @@ __global__ void __launch_bounds__(256) k_ref_mem(RefGeo r, const double *__restr
     const int k = blockIdx.z + (L::D == 3 ? r.g : 0);
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (i < r.lx - r.g) {
-        const long n = (long)i + (long)r.lx * ((long)j + (long)r.ly * k);
+        const long n = (long)i + (long)r.px * ((long)j + (long)r.ly * k);
         if (usr[n] == 1) {
             static_for<0, Q>([&](auto I) {
                 constexpr int ii = I;
-                const long nb = n + L::ex(ii) + (long)r.lx * (L::ey(ii) + (long)r.ly * L::ez(ii));
+                const long nb = n + L::ex(ii) + (long)r.px * (L::ey(ii) + (long)r.ly * L::ez(ii));
                 if (usr[nb] == 0) {
                     const double m = f[(size_t)ii * r.plane + n] + f[(size_t)L::opp(ii) * r.plane + nb];
                     fx += m * (double)L::ex(ii);
@@ -233,7 +234,7 @@ __global__ void __launch_bounds__(256) k_ref_fused(RefGeo r, const double *__res
     using L = Lat<Q>;
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
     if (i >= r.lx) return;
-    const size_t n = (size_t)i + (size_t)r.lx * ((size_t)j + (size_t)r.ly * k);
+    const size_t n = (size_t)i + (size_t)r.px * ((size_t)j + (size_t)r.ly * k);
     const int g = r.g;
     const bool interior = i >= g && i < r.lx - g && j >= g && j < r.ly - g &&
                           (L::D == 2 || (k >= g && k < r.lz - g));
@@ -245,7 +246,7 @@ __global__ void __launch_bounds__(256) k_ref_fused(RefGeo r, const double *__res
     double p = 0.0, u = 0.0, v = 0.0, w = 0.0;
     static_for<0, Q>([&](auto I) {
         constexpr int ii = I;
-        const long src = (long)n - (L::ex(ii) + (long)r.lx * (L::ey(ii) + (long)r.ly * L::ez(ii)));
+        const long src = (long)n - (L::ex(ii) + (long)r.px * (L::ey(ii) + (long)r.ly * L::ez(ii)));
         fv[ii] = f[(size_t)ii * r.plane + src];
         if (fpre) fpre[(size_t)ii * r.plane + n] = fv[ii];
         p = p + fv[ii] / 3.0;

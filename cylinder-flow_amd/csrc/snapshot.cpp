@@ -210,7 +210,7 @@ extern "C" int lbm_snapshot_async(lbm_ctx *c, const char *path, int n) {
                 for (int kk = 0; kk < glz; kk++)
                     for (int j = 0; j < gly; j++)
                         for (int i = 0; i < glx; i++, o++)
-                            strip[k * N + o] = pinned[k * plane + (size_t)(i + g) + (size_t)rg.lx * ((size_t)(j + g) + (size_t)rg.ly * (kk + gz))];
+                            strip[k * N + o] = pinned[k * plane + (size_t)(i + g) + (size_t)rg.px * ((size_t)(j + g) + (size_t)rg.ly * (kk + gz))];
                 fld[k] = strip.data() + k * N;
             }
         } else {
