@@ -96,7 +96,7 @@ struct RefGeo {
 };
 hipError_t ref_init(const RefGeo &r, double *f, double *ft, double *mac, uint8_t *usr, const int32_t *ran,
                     int body, double cx, double cy, double cz, double r2, uint64_t seed, hipStream_t s);
-hipError_t ref_fills(const RefGeo &r, double *f, hipStream_t s);
+hipError_t ref_fills(const RefGeo &r, double *f, double *ft, hipStream_t s);
 hipError_t ref_usrbc(const RefGeo &r, double *f, const uint8_t *usr, hipStream_t s);
 hipError_t ref_mem(const RefGeo &r, const double *f, const uint8_t *usr, double *partials, int nblk,
                    double *hist, int rec, long step, hipStream_t s);

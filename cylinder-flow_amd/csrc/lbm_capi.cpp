@@ -663,7 +663,7 @@ static int step_n(lbm_ctx *c, int n, hipEvent_t *evs = nullptr) {
         for (int k = 0; k < n; k++) {
             c->step++;
             const bool force = fi > 0 && c->step % fi == 0;
-            HIP_TRY(c, ref_fills(c->rg, c->rf, c->stream));
+            HIP_TRY(c, ref_fills(c->rg, c->rf, c->rft, c->stream));
             HIP_TRY(c, ref_usrbc(c->rg, c->rf, c->rusr, c->stream));
             if (force) HIP_TRY(c, ref_mem(c->rg, c->rf, c->rusr, c->partials, c->nslots, c->hist, c->nrec++, c->step, c->stream));
             HIP_TRY(c, ref_fused(c->rg, c->rf, c->rft, k == n - 1 ? c->rmac : nullptr, c->rusr, c->rprof,

@@ -82,9 +82,19 @@ __global__ void __launch_bounds__(256) k_ref_init(RefGeo r, RefBody b, double *_
     mac[2 * r.plane + n] = 0.0;
     mac[3 * r.plane + n] = 0.0;
     static_for<0, Q>([&](auto I) {
-        f[(size_t)I * r.plane + n] = ref_feq<Q, I>(0.0, u, 0.0, 0.0);
-        ft[(size_t)I * r.plane + n] = 0.0;
+        const double v = ref_feq<Q, I>(0.0, u, 0.0, 0.0);
+        f[(size_t)I * r.plane + n] = v;
+        ft[(size_t)I * r.plane + n] = v;   // the ghosts of both buffers start equal (interior: overwritten)
     });
+}
+
+// A ghost fill writes both buffers: ghost slots then hold the same values in f and ft, so the
+// A/B swap after the fused kernel keeps them (the reference's f keeps its ghosts: its
+// stream2 copies the interior only), and the fused kernel need not copy them.
+__device__ __forceinline__ void copy2(double *__restrict__ f, double *__restrict__ ft, size_t dst, size_t src) {
+    const double v = f[src];
+    f[dst] = v;
+    ft[dst] = v;
 }
 
 #define RIDX(r, ii, i, j, k) ((size_t)(ii) * (r).plane + (size_t)((long)(i) + (long)(r).px * ((long)(j) + (long)(r).ly * (long)(k))))
@@ -92,7 +102,7 @@ __global__ void __launch_bounds__(256) k_ref_init(RefGeo r, RefBody b, double *_
 // 3D fills, one kernel per face: 3D MPI+KOKKOS+MPIIO/lbm.cpp:278-347.
 // face: 0 front (y lo), 1 back (y hi), 2 bottom (z lo), 3 top (z hi), 4 inlet, 5 outlet
 template <int FACE>
-__global__ void __launch_bounds__(256) k_ref_fill3(RefGeo r, double *__restrict__ f) {
+__global__ void __launch_bounds__(256) k_ref_fill3(RefGeo r, double *__restrict__ f, double *__restrict__ ft) {
     using L = Lat<27>;
     const int a = blockIdx.x * blockDim.x + threadIdx.x + r.g - 1;   // first in-face coordinate
     const int b = blockIdx.y + r.g - 1;                              // second in-face coordinate
@@ -105,16 +115,16 @@ __global__ void __launch_bounds__(256) k_ref_fill3(RefGeo r, double *__restrict_
         constexpr int ii = I;
         constexpr int ex = L::ex(ii), ey = L::ey(ii), ez = L::ez(ii), bb = L::opp(ii);
         if constexpr (FACE == 0 && ey > 0)
-            f[RIDX(r, ii, a, g - 1, b)] = f[RIDX(r, bb, a + 2 * ex, g + 1, b + 2 * ez)];
+            copy2(f, ft, RIDX(r, ii, a, g - 1, b), RIDX(r, bb, a + 2 * ex, g + 1, b + 2 * ez));
         if constexpr (FACE == 1 && ey < 0)
-            f[RIDX(r, ii, a, le1, b)] = f[RIDX(r, bb, a + 2 * ex, le1 - 2, b + 2 * ez)];
+            copy2(f, ft, RIDX(r, ii, a, le1, b), RIDX(r, bb, a + 2 * ex, le1 - 2, b + 2 * ez));
         if constexpr (FACE == 2 && ez > 0)
-            f[RIDX(r, ii, a, b, g - 1)] = f[RIDX(r, bb, a + 2 * ex, b + 2 * ey, g + 1)];
+            copy2(f, ft, RIDX(r, ii, a, b, g - 1), RIDX(r, bb, a + 2 * ex, b + 2 * ey, g + 1));
         if constexpr (FACE == 3 && ez < 0)
-            f[RIDX(r, ii, a, b, le2)] = f[RIDX(r, bb, a + 2 * ex, b + 2 * ey, le2 - 2)];
-        if constexpr (FACE == 4 && ex > 0) f[RIDX(r, ii, g - 1, a, b)] = f[RIDX(r, ii, g, a, b)];
+            copy2(f, ft, RIDX(r, ii, a, b, le2), RIDX(r, bb, a + 2 * ex, b + 2 * ey, le2 - 2));
+        if constexpr (FACE == 4 && ex > 0) copy2(f, ft, RIDX(r, ii, g - 1, a, b), RIDX(r, ii, g, a, b));
         if constexpr (FACE == 5 && ex < 0)
-            f[RIDX(r, ii, le0, a, b)] = f[RIDX(r, ii, le0 + ex, a + ey, b + ez)];
+            copy2(f, ft, RIDX(r, ii, le0, a, b), RIDX(r, ii, le0 + ex, a + ey, b + ez));
     });
 }
 
@@ -122,33 +132,33 @@ __global__ void __launch_bounds__(256) k_ref_fill3(RefGeo r, double *__restrict_
 // stream4 top).  Columns i-2 = -1 and i+2 = lx wrap through the flat index, as in the
 // reference's LayoutLeft views.
 template <int FACE>
-__global__ void __launch_bounds__(256) k_ref_fill2(RefGeo r, double *__restrict__ f) {
+__global__ void __launch_bounds__(256) k_ref_fill2(RefGeo r, double *__restrict__ f, double *__restrict__ ft) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int lx = r.lx, ly = r.ly;
     if (FACE <= 1) {
         if (t >= ly - 1) return;
         const int j = t;
         if (FACE == 0) {
-            f[RIDX(r, 1, 1, j, 0)] = f[RIDX(r, 1, 2, j, 0)];
-            f[RIDX(r, 5, 1, j, 0)] = f[RIDX(r, 5, 2, j, 0)];
-            f[RIDX(r, 8, 1, j, 0)] = f[RIDX(r, 8, 2, j, 0)];
+            copy2(f, ft, RIDX(r, 1, 1, j, 0), RIDX(r, 1, 2, j, 0));
+            copy2(f, ft, RIDX(r, 5, 1, j, 0), RIDX(r, 5, 2, j, 0));
+            copy2(f, ft, RIDX(r, 8, 1, j, 0), RIDX(r, 8, 2, j, 0));
         } else {
-            f[RIDX(r, 3, lx - 2, j, 0)] = f[RIDX(r, 3, lx - 3, j, 0)];
-            f[RIDX(r, 7, lx - 2, j, 0)] = f[RIDX(r, 7, lx - 3, j - 1, 0)];
-            f[RIDX(r, 6, lx - 2, j, 0)] = f[RIDX(r, 6, lx - 3, j + 1, 0)];
+            copy2(f, ft, RIDX(r, 3, lx - 2, j, 0), RIDX(r, 3, lx - 3, j, 0));
+            copy2(f, ft, RIDX(r, 7, lx - 2, j, 0), RIDX(r, 7, lx - 3, j - 1, 0));
+            copy2(f, ft, RIDX(r, 6, lx - 2, j, 0), RIDX(r, 6, lx - 3, j + 1, 0));
         }
     } else {
         if (t >= lx - 1) return;
         const int i = t;
         if (r.variant == 1 && (i < 2 || i >= lx - 2)) return;   // Kokkos/lbm.cpp:158: i in [2, lx-2)
         if (FACE == 2) {
-            f[RIDX(r, 2, i, 1, 0)] = f[RIDX(r, 4, i, 3, 0)];
-            f[RIDX(r, 5, i, 1, 0)] = f[RIDX(r, 7, i + 2, 3, 0)];
-            f[RIDX(r, 6, i, 1, 0)] = f[RIDX(r, 8, i - 2, 3, 0)];
+            copy2(f, ft, RIDX(r, 2, i, 1, 0), RIDX(r, 4, i, 3, 0));
+            copy2(f, ft, RIDX(r, 5, i, 1, 0), RIDX(r, 7, i + 2, 3, 0));
+            copy2(f, ft, RIDX(r, 6, i, 1, 0), RIDX(r, 8, i - 2, 3, 0));
         } else {
-            f[RIDX(r, 4, i, ly - 2, 0)] = f[RIDX(r, 2, i, ly - 4, 0)];
-            f[RIDX(r, 7, i, ly - 2, 0)] = f[RIDX(r, 5, i - 2, ly - 4, 0)];
-            f[RIDX(r, 8, i, ly - 2, 0)] = f[RIDX(r, 6, i + 2, ly - 4, 0)];
+            copy2(f, ft, RIDX(r, 4, i, ly - 2, 0), RIDX(r, 2, i, ly - 4, 0));
+            copy2(f, ft, RIDX(r, 7, i, ly - 2, 0), RIDX(r, 5, i - 2, ly - 4, 0));
+            copy2(f, ft, RIDX(r, 8, i, ly - 2, 0), RIDX(r, 6, i + 2, ly - 4, 0));
         }
     }
 }
@@ -238,10 +248,7 @@ __global__ void __launch_bounds__(256) k_ref_fused(RefGeo r, const double *__res
     const int g = r.g;
     const bool interior = i >= g && i < r.lx - g && j >= g && j < r.ly - g &&
                           (L::D == 2 || (k >= g && k < r.lz - g));
-    if (!interior) {
-        static_for<0, Q>([&](auto I) { ft[(size_t)I * r.plane + n] = f[(size_t)I * r.plane + n]; });
-        return;
-    }
+    if (!interior) return;   // ghost slots are equal in both buffers (copy2)
     double fv[Q];
     double p = 0.0, u = 0.0, v = 0.0, w = 0.0;
     static_for<0, Q>([&](auto I) {
@@ -341,20 +348,20 @@ hipError_t ref_init(const RefGeo &r, double *f, double *ft, double *mac, uint8_t
     return hipGetLastError();
 }
 
-hipError_t ref_fills(const RefGeo &r, double *f, hipStream_t s) {
+hipError_t ref_fills(const RefGeo &r, double *f, double *ft, hipStream_t s) {
     if (r.D == 3) {
         const int nI = r.lx - 2 * (r.g - 1), nJ = r.ly - 2 * (r.g - 1), nK = r.lz - 2 * (r.g - 1);
-        hipLaunchKernelGGL(k_ref_fill3<0>, dim3((nI + 255) / 256, nK), dim3(256), 0, s, r, f);
-        hipLaunchKernelGGL(k_ref_fill3<1>, dim3((nI + 255) / 256, nK), dim3(256), 0, s, r, f);
-        hipLaunchKernelGGL(k_ref_fill3<2>, dim3((nI + 255) / 256, nJ), dim3(256), 0, s, r, f);
-        hipLaunchKernelGGL(k_ref_fill3<3>, dim3((nI + 255) / 256, nJ), dim3(256), 0, s, r, f);
-        hipLaunchKernelGGL(k_ref_fill3<4>, dim3((nJ + 255) / 256, nK), dim3(256), 0, s, r, f);
-        hipLaunchKernelGGL(k_ref_fill3<5>, dim3((nJ + 255) / 256, nK), dim3(256), 0, s, r, f);
+        hipLaunchKernelGGL(k_ref_fill3<0>, dim3((nI + 255) / 256, nK), dim3(256), 0, s, r, f, ft);
+        hipLaunchKernelGGL(k_ref_fill3<1>, dim3((nI + 255) / 256, nK), dim3(256), 0, s, r, f, ft);
+        hipLaunchKernelGGL(k_ref_fill3<2>, dim3((nI + 255) / 256, nJ), dim3(256), 0, s, r, f, ft);
+        hipLaunchKernelGGL(k_ref_fill3<3>, dim3((nI + 255) / 256, nJ), dim3(256), 0, s, r, f, ft);
+        hipLaunchKernelGGL(k_ref_fill3<4>, dim3((nJ + 255) / 256, nK), dim3(256), 0, s, r, f, ft);
+        hipLaunchKernelGGL(k_ref_fill3<5>, dim3((nJ + 255) / 256, nK), dim3(256), 0, s, r, f, ft);
     } else {
-        hipLaunchKernelGGL(k_ref_fill2<0>, dim3((r.ly + 255) / 256), dim3(256), 0, s, r, f);
-        hipLaunchKernelGGL(k_ref_fill2<1>, dim3((r.ly + 255) / 256), dim3(256), 0, s, r, f);
-        hipLaunchKernelGGL(k_ref_fill2<2>, dim3((r.lx + 255) / 256), dim3(256), 0, s, r, f);
-        hipLaunchKernelGGL(k_ref_fill2<3>, dim3((r.lx + 255) / 256), dim3(256), 0, s, r, f);
+        hipLaunchKernelGGL(k_ref_fill2<0>, dim3((r.ly + 255) / 256), dim3(256), 0, s, r, f, ft);
+        hipLaunchKernelGGL(k_ref_fill2<1>, dim3((r.ly + 255) / 256), dim3(256), 0, s, r, f, ft);
+        hipLaunchKernelGGL(k_ref_fill2<2>, dim3((r.lx + 255) / 256), dim3(256), 0, s, r, f, ft);
+        hipLaunchKernelGGL(k_ref_fill2<3>, dim3((r.lx + 255) / 256), dim3(256), 0, s, r, f, ft);
     }
     return hipGetLastError();
 }
