@@ -14,7 +14,9 @@
 //   3. mu_m(a) = (collision + partial bounce-back)^T lambda + alpha_m dFx/dG, then the
 //      transposes of the regularisation and of Zou-He at inlet nodes; accumulate dJ/ds(a)
 //      and dJ/du_in(row).
-// Traffic per node: Q reads of H, Q reads of mu_{m+1}, Q writes of mu_m.
+// Traffic per node: Q reads of H, Q reads of mu_{m+1}, Q writes of mu_m.  H is a population
+// buffer of the forward (read through its pull); the mu arrays are planar, one plane per
+// direction: mu[i*plane + node].
 #include "fused_common.hpp"
 
 namespace lbm {

@@ -5,9 +5,11 @@
 //  * k_fast_v runs the packed path on every lane of every row: each lane owns V consecutive x
 //    nodes, so each of the 27 pulls and 27 stores is one V*sizeof(storage)-byte access (a
 //    dword for two FP16 nodes); the source row of a direction is a scalar and only three lane
-//    offsets exist (x-1, x, x+1).  For two 16-bit nodes per lane, interior rows load every
-//    direction aligned and build the x-1 / x+1 pairs from the neighbouring lane's dword
-//    (aligned_pull), so a wave's request spans two cache lines instead of three.  It applies
+//    offsets exist (x-1, x, x+1).  The populations of a row are row-interleaved (direction i
+//    of row r at i*nxp + r*Q*nxp), so a row's update reads and writes Q*nxp contiguous
+//    elements plus the source rows' stretches.  Optionally (aligned_pull), 16-bit interior
+//    rows load every direction aligned and build the x-1 / x+1 pairs from the neighbouring
+//    lane's dword, so a wave's request spans two cache lines instead of three.  It applies
 //    walls, periodic y/z, the inlet and the outlet, but knows nothing of the mask: solid nodes
 //    get values that no fluid node ever reads;
 //  * before it, k_link_pairs (one lane per link) or, on steps that record the force, k_link_v
@@ -130,9 +132,9 @@ __device__ __forceinline__ void store_pack(char *p, const Pack<T, V> &r) {
     __builtin_nontemporal_store(u, (U *)p);
 }
 
-// Byte offsets, for one storage type and geometry, of the direction planes (dst) and of each
-// direction's nominal source row relative to the destination row, x shift excluded (src):
-//   dst[i] = i * plane * sizeof(T),   src[i] = dst[i] - (e_iy + ny e_iz) * nxp * sizeof(T)
+// Byte offsets, for one storage type and geometry, of direction i's row relative to the start
+// of a row's populations (dst) and of its nominal source row, x shift excluded (src):
+//   dst[i] = i * ds * sizeof(T),   src[i] = dst[i] - (e_iy + ny e_iz) * rs * sizeof(T)
 // computed on the host per launch, so a block addresses direction i with one scalar add.
 struct PlaneTab {
     int64_t dst[27];
@@ -144,8 +146,8 @@ inline PlaneTab plane_tab(const Geo &g) {
     using L = Lat<Q>;
     PlaneTab t{};
     for (int i = 0; i < Q; i++) {
-        t.dst[i] = (int64_t)i * (int64_t)g.plane * (int64_t)sizeof(T);
-        t.src[i] = t.dst[i] - ((int64_t)L::ey(i) + (int64_t)g.ny * L::ez(i)) * g.nxp * (int64_t)sizeof(T);
+        t.dst[i] = (int64_t)i * (int64_t)g.ds * (int64_t)sizeof(T);
+        t.src[i] = t.dst[i] - ((int64_t)L::ey(i) + (int64_t)g.ny * L::ez(i)) * (int64_t)g.rs * (int64_t)sizeof(T);
     }
     return t;
 }
@@ -202,7 +204,7 @@ template <int Q, class S, int V, bool MACROS, bool ALIGNED>
 __device__ __forceinline__ void packed_nodes(const typename S::T *__restrict__ A, char *__restrict__ Brow,
                                              const double *__restrict__ uin, double *__restrict__ macros,
                                              const FusedArgs &a, const PlaneTab &tab, int y, int z, int x0,
-                                             size_t rowoff) {
+                                             size_t rowoff, size_t poprow) {
     using L = Lat<Q>;
     using T = typename S::T;
     const Geo &g = a.g;
@@ -210,7 +212,7 @@ __device__ __forceinline__ void packed_nodes(const typename S::T *__restrict__ A
     const uint32_t b0 = (uint32_t)x0 * sizeof(T);
     const uint32_t bP = first ? b0 : b0 - (uint32_t)sizeof(T);   // e_x = +1 pulls x-1
     const uint32_t bM = last ? b0 : b0 + (uint32_t)sizeof(T);    // e_x = -1 pulls x+1
-    const char *const Arow = (const char *)A + rowoff * sizeof(T);
+    const char *const Arow = (const char *)A + poprow * sizeof(T);
     const bool interior = (unsigned)(y - 1) < (unsigned)(g.ny - 2) &&
                           (L::D == 2 || (unsigned)(z - 1) < (unsigned)(g.nz - 2));
     Pack<T, V> r[Q];
@@ -246,7 +248,7 @@ __device__ __forceinline__ void packed_nodes(const typename S::T *__restrict__ A
                     else bb = true;
                 }
             }
-            const int64_t shift = (int64_t)g.nxp * (dy + (int64_t)g.ny * dz) * (int64_t)sizeof(T);
+            const int64_t shift = (int64_t)g.rs * (dy + (int64_t)g.ny * dz) * (int64_t)sizeof(T);
             const char *rowp = bb ? Arow + tab.dst[o] : Arow + tab.src[i] + shift;
             const uint32_t bo = bb ? b0 : (ex > 0 ? bP : (ex < 0 ? bM : b0));
             r[i] = load_pack<T, V>(rowp + bo);
@@ -304,8 +306,9 @@ __global__ void __launch_bounds__(256)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;   // lane index within the x row
     if (t * V >= g.nxp) return;
     const int y = blockIdx.y, z = blockIdx.z;
-    const size_t rowoff = (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
-    packed_nodes<Q, S, V, MACROS, ALIGNED>(A, (char *)(B + rowoff), uin, macros, a, tab, y, z, t * V, rowoff);
+    const size_t row = (size_t)y + (size_t)g.ny * z;
+    const size_t rowoff = (size_t)g.nxp * row, poprow = g.rs * row;
+    packed_nodes<Q, S, V, MACROS, ALIGNED>(A, (char *)(B + poprow), uin, macros, a, tab, y, z, t * V, rowoff, poprow);
 }
 
 // Pre-applied mask bounce-back, before k_fast_v on the same stream.  Half-way bounce-back
@@ -331,7 +334,7 @@ __global__ void __launch_bounds__(256)
     T val[Q];
     static_for<1, Q>([&](auto I) {   // every bounced value first, so the loads are in flight together
         constexpr int i = I;
-        if ((lk >> i) & 1u) val[i] = A[(size_t)L::opp(i) * g.plane + node];
+        if ((lk >> i) & 1u) val[i] = A[pel(g, L::opp(i), (uint32_t)x, row)];
     });
     double fx = 0.0, fy = 0.0, fz = 0.0;
     static_for<1, Q>([&](auto I) {
@@ -343,8 +346,7 @@ __global__ void __launch_bounds__(256)
             int sy = y - ey, sz = z - ez;
             if constexpr (ey != 0) sy = sy < 0 ? sy + g.ny : (sy >= g.ny ? sy - g.ny : sy);
             if constexpr (ez != 0) sz = sz < 0 ? sz + g.nz : (sz >= g.nz ? sz - g.nz : sz);
-            const uint32_t src = (uint32_t)(x - ex) + (uint32_t)g.nxp * ((uint32_t)sy + (uint32_t)g.ny * (uint32_t)sz);
-            A[(size_t)i * g.plane + src] = val[i];
+            A[pel(g, i, (uint32_t)(x - ex), (uint32_t)sy + (uint32_t)g.ny * (uint32_t)sz)] = val[i];
             if constexpr (FORCE) {
                 const double m = 2.0 * (double)S::template decode<i, L>(val[i], a.sp.off[o]);
                 if constexpr (ex != 0) fx -= ex * m;
@@ -401,7 +403,7 @@ __global__ void __launch_bounds__(256)
     float sf = 0.f;
     if constexpr (SFRAC) sf = sfrac[node];
     collide<Q, FORCE, SFRAC>(gv, sf, a.omega, c, s, u, v, w, fx, fy, fz);
-    char *const Bn = (char *)B + (size_t)node * sizeof(T);
+    char *const Bn = (char *)B + pel(g, 0, (uint32_t)x, row) * sizeof(T);
     static_for<0, Q>([&](auto I) {
         constexpr int i = I;
         __builtin_nontemporal_store(S::template enc<i, L>(c[i], a.sp), (T *)(Bn + tab.dst[i]));

@@ -162,9 +162,9 @@ __device__ __forceinline__ void block_sum3(double &a, double &b, double &c, doub
 // Which sources are solid: LINKS = false gathers the flags of the Q sources when `near`;
 // LINKS = true takes them from `lk` (bit i: direction i pulls from a solid node), computed
 // once on the host with the same rules, so no flag is read.
-// `node` must be a valid element of the plane even when the lane is not live; every load
-// below is issued unconditionally from a valid address (no data-dependent branches), so the
-// Q loads of a lane are in flight together.
+// `node` (= x + nxp*row) must be a valid node even when the lane is not live; every load below
+// is issued unconditionally from a valid address (no data-dependent branches), so the Q loads
+// of a lane are in flight together.
 template <int Q, class S, bool FORCE, bool LINKS = false>
 __device__ __forceinline__ void pull(const typename S::T *__restrict__ A, const uint8_t *__restrict__ flags,
                                      const Geo &g, const StoreParams &sp, int x, int y, int z, uint32_t node,
@@ -173,8 +173,9 @@ __device__ __forceinline__ void pull(const typename S::T *__restrict__ A, const 
     using L = Lat<Q>;
     using T = typename S::T;
     using C = typename S::C;
-    uint32_t src[Q];
+    uint32_t src[Q], srow[Q];   // source node and its row
     bool zh[Q], wall[Q], solid[Q];
+    const uint32_t row = (uint32_t)y + (uint32_t)g.ny * (uint32_t)z;
     static_for<0, Q>([&](auto I) {
         constexpr int i = I;
         constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i);
@@ -204,7 +205,8 @@ __device__ __forceinline__ void pull(const typename S::T *__restrict__ A, const 
         zh[i] = z_;
         wall[i] = w_ && !z_;
         const bool plain = live && !z_ && !w_;
-        src[i] = plain ? (uint32_t)sx + (uint32_t)g.nxp * ((uint32_t)sy + (uint32_t)g.ny * (uint32_t)sz) : node;
+        srow[i] = plain ? (uint32_t)sy + (uint32_t)g.ny * (uint32_t)sz : row;
+        src[i] = plain ? (uint32_t)sx + (uint32_t)g.nxp * srow[i] : node;
         solid[i] = false;
     });
     if constexpr (LINKS) {
@@ -219,7 +221,9 @@ __device__ __forceinline__ void pull(const typename S::T *__restrict__ A, const 
         constexpr int i = I;
         constexpr int o = L::opp(i);
         const bool bb = wall[i] || solid[i];
-        raw[i] = bb ? A[(size_t)o * g.plane + node] : A[(size_t)i * g.plane + src[i]];
+        // element i*ds + row*rs + x, with node = x + nxp*row
+        raw[i] = bb ? A[(size_t)o * g.ds + node + (size_t)row * (g.rs - g.nxp)]
+                    : A[(size_t)i * g.ds + src[i] + (size_t)srow[i] * (g.rs - g.nxp)];
     });
     static_for<0, Q>([&](auto I) {
         constexpr int i = I;

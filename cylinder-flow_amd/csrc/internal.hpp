@@ -13,14 +13,25 @@
 
 namespace lbm {
 
-// Device layout of the plain path: SoA, one plane per direction, x fastest, rows padded
-// to `nxp` (a multiple of 64 elements) so every wavefront row segment starts aligned.
-//   element (i, x, y, z) = i*plane + x + nxp*(y + ny*z),   plane = nxp*ny*nz
+// Device layout of the plain path.  Node arrays (flags, moments, solid fraction) are x fastest
+// with rows padded to `nxp` (a multiple of 64 elements), so every wavefront row segment starts
+// aligned:  node (x, y, z) = x + nxp*row,  row = y + ny*z,  plane = nxp*ny*nz.
+// A population buffer holds, per row, the Q direction rows one after another (row-interleaved):
+//   element (i, x, row) = i*ds + row*rs + x,   ds = nxp, rs = Q*nxp   (Q*plane elements)
+// so one row's update reads and writes Q*nxp*sizeof(T) contiguous bytes (plus the source rows'
+// stretches) instead of Q strips one plane apart.  ds = plane, rs = nxp gives the layout with
+// one plane per direction, kept selectable (LBM_DIRECTION_PLANES) for comparison.
 struct Geo {
     int nx, ny, nz, nxp;
     size_t plane;
+    size_t ds, rs;
     int xper, yper, zper;
 };
+
+// population element of direction i at (x, row)
+__host__ __device__ __forceinline__ size_t pel(const Geo &g, int i, uint32_t x, uint32_t row) {
+    return (size_t)i * g.ds + (size_t)row * g.rs + x;
+}
 
 struct FusedArgs {
     Geo g;

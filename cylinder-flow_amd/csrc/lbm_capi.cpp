@@ -136,12 +136,7 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_ctx **out) {
     if (cfg->ncoef < 0 || cfg->ncoef > 8) return set_err(nullptr, LBM_ERR_INVALID, "ncoef must be in [0, 8]");
     if (cfg->force_interval < 0) return set_err(nullptr, LBM_ERR_INVALID, "force_interval must be >= 0");
     const int nxp = (cfg->nx + 63) / 64 * 64;
-    // direction planes are `plane` elements apart: the nodes of one plane plus an optional gap
-    // (LBM_PLANE_PAD elements, rounded up to 64), so that the Q planes need not sit a power of
-    // two apart in memory
-    size_t pad = 0;
-    if (const char *env = std::getenv("LBM_PLANE_PAD")) pad = ((size_t)std::max(0L, std::atol(env)) + 63) / 64 * 64;
-    const size_t plane = (size_t)nxp * cfg->ny * nz + pad;
+    const size_t plane = (size_t)nxp * cfg->ny * nz;
     if (plane >= (size_t)1 << 32) return set_err(nullptr, LBM_ERR_INVALID, "domain too large: a direction plane must stay below 2^32 nodes");
 
     lbm_ctx *c = new lbm_ctx();
@@ -155,6 +150,11 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_ctx **out) {
     c->geo.nz = nz;
     c->geo.nxp = nxp;
     c->geo.plane = plane;
+    // populations row-interleaved (the Q direction rows of a row side by side); the layout with
+    // one plane per direction (LBM_DIRECTION_PLANES) is kept for comparison, same results
+    const bool planes = std::getenv("LBM_DIRECTION_PLANES") != nullptr;
+    c->geo.ds = planes ? plane : (size_t)nxp;
+    c->geo.rs = planes ? (size_t)nxp : (size_t)Q * nxp;
     const bool plain = cfg->bc_mode == LBM_BC_PLAIN;
     c->geo.xper = plain && cfg->periodic[0];
     c->geo.yper = plain && cfg->periodic[1];
@@ -169,14 +169,17 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_ctx **out) {
         const bool ok = c->prec == LBM_FP32 ? (v == 1 || v == 2) : c->prec != LBM_FP64 && (v == 2 || v == 4);
         if (ok) c->V = v;
     }
-    // 16-bit storage with two nodes per lane pulls with aligned loads and lane shifts;
-    // LBM_UNALIGNED_PULL selects the equivalent unaligned loads (same results, for comparison).
-    c->aligned_pull = std::getenv("LBM_UNALIGNED_PULL") == nullptr;
-    // Blocks of the packed kernel resident per CU: FP32 runs fastest at 3 (3 waves per SIMD,
-    // measured on the 512x256x256 case: 69.5 % of HBM peak against 67.8 % uncapped), the
-    // 16-bit formats uncapped.  The cap is a dynamic LDS allocation the kernel does not use.
-    // LBM_FAST_BLOCKS_PER_CU overrides it (0: no cap).
-    int bpc = c->prec == LBM_FP32 ? 3 : 0;
+    // 16-bit storage with two nodes per lane pulls with one (2-byte aligned) dword load per
+    // direction; LBM_ALIGNED_PULL selects aligned loads with lane shifts instead (same
+    // results; faster with one plane per direction, slower with the row-interleaved layout).
+    c->aligned_pull = std::getenv("LBM_ALIGNED_PULL") != nullptr;
+    // Blocks of the packed kernel resident per CU (256-lane blocks, measured on the 512x256x256
+    // cylinder with row-interleaved populations): FP32 runs fastest at 3 (3 waves per SIMD:
+    // 71.5 % of HBM peak per step against 70.8 % uncapped), FP16/BF16 at 4 (69.9 % against
+    // 67.3 % for FP16).  The cap is a dynamic LDS allocation the kernel does not use.
+    // LBM_FAST_BLOCKS_PER_CU overrides it (0: no cap).  Rows of fewer than 256 lanes (narrow
+    // domains) keep the same number of waves per CU.
+    int bpc = c->prec == LBM_FP32 ? 3 : c->prec == LBM_FP64 ? 0 : 4;
     if (const char *env = std::getenv("LBM_FAST_BLOCKS_PER_CU")) bpc = std::max(0, std::atoi(env));
     hipError_t e = hipSuccess;
     if (bpc > 0) {
@@ -188,9 +191,13 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_ctx **out) {
             delete c;
             return LBM_ERR_HIP;
         }
-        // midway between lds/(bpc+1) and lds/bpc, so allocation granularity cannot tip the count
-        const unsigned per = (unsigned)(2LL * lds / (2 * bpc + 1)) / 1024u * 1024u;
-        if (per > (unsigned)lds / (unsigned)(bpc + 1) && per <= 65536u) c->fast_lds = per;
+        // the count is of 4-wave (256-lane) blocks: rows narrower than 256 lanes launch smaller
+        // blocks, and proportionally more of them may stay resident
+        const int lanes = nxp / c->V, wpb = (std::min(lanes, 256) + 63) / 64;
+        const int blocks = std::max(1, bpc * 4 / wpb);
+        // midway between lds/(blocks+1) and lds/blocks, so allocation granularity cannot tip the count
+        const unsigned per = (unsigned)(2LL * lds / (2 * blocks + 1)) / 512u * 512u;
+        if (per > (unsigned)lds / (unsigned)(blocks + 1) && per <= 65536u) c->fast_lds = per;
     }
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -499,7 +506,7 @@ static int init_plain(lbm_ctx *c) {
         }
         // The copies again, one entry per link (steps without force): element indices of the
         // bounced value h_opp(i)(x) and of the solid slot (x - e_i, i), sorted by the
-        // destination, so a wave's lanes share direction planes and rows.
+        // destination, so a wave's lanes share rows.
         if (c->nlink && (uint64_t)Q * g.plane <= UINT32_MAX) {
             std::vector<std::pair<uint32_t, uint32_t>> pairs;   // (destination, source)
             for (size_t j = 0; j < link.size(); j++) {
@@ -508,8 +515,8 @@ static int init_plain(lbm_ctx *c) {
                 for (int i = 1; i < Q; i++) {
                     if (!((linklk[j] >> i) & 1u)) continue;
                     const int sy = (y - e[3 * i + 1] + g.ny) % g.ny, sz = (z - e[3 * i + 2] + g.nz) % g.nz;
-                    const size_t src = (size_t)(x - e[3 * i]) + (size_t)g.nxp * ((size_t)sy + (size_t)g.ny * sz);
-                    pairs.emplace_back((uint32_t)((size_t)i * g.plane + src), (uint32_t)((size_t)opp[i] * g.plane + node));
+                    const size_t src = pel(g, i, (uint32_t)(x - e[3 * i]), (uint32_t)(sy + g.ny * sz));
+                    pairs.emplace_back((uint32_t)src, (uint32_t)pel(g, opp[i], (uint32_t)x, row));
                 }
             }
             std::sort(pairs.begin(), pairs.end());

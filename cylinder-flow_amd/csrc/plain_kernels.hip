@@ -57,8 +57,9 @@ __global__ void __launch_bounds__(512) k_init(typename S::T *__restrict__ A, typ
         constexpr int i = I;
         const double f = live ? feq_ref<Q, i>(0.0, u, 0.0, 0.0) : 0.0;
         const T h = S::encode(f, (float)(1.0 / L::w(i)), a.sp.offw[i]);
-        A[(size_t)i * g.plane + node] = h;
-        B[(size_t)i * g.plane + node] = h;
+        const size_t e = pel(g, i, (uint32_t)x, (uint32_t)y + (uint32_t)g.ny * (uint32_t)z);
+        A[e] = h;
+        B[e] = h;
     });
 }
 
@@ -74,9 +75,11 @@ __global__ void __launch_bounds__(512)
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     const int y = blockIdx.y, z = blockIdx.z;
     const bool active = x < g.nx;
-    // element offset inside one direction plane (a plane is < 2^32 elements)
-    // lanes past the row (x >= nx) use the row's first element as a valid, unused address
-    const uint32_t node = (uint32_t)(active ? x : 0) + (uint32_t)g.nxp * ((uint32_t)y + (uint32_t)g.ny * (uint32_t)z);
+    // node offset (a plane is < 2^32 nodes); lanes past the row (x >= nx) use the row's first
+    // node as a valid, unused address.  Its populations start at element pnode (+ i*ds).
+    const uint32_t row = (uint32_t)y + (uint32_t)g.ny * (uint32_t)z;
+    const uint32_t node = (uint32_t)(active ? x : 0) + (uint32_t)g.nxp * row;
+    const size_t pnode = node + (size_t)row * (g.rs - g.nxp);
     const uint8_t fl = active ? flags[node] : (uint8_t)FL_SOLID;
     const bool live = fl != FL_SOLID;
     const bool near = fl == FL_NEAR;
@@ -117,7 +120,7 @@ __global__ void __launch_bounds__(512)
             const double feq = feq_ref<Q, i>(p, u, v, w);
             double c = gv[i] - (gv[i] - feq) / a.tau;
             if constexpr (SFRAC) c = (1.0 - sf) * c + sf * gv[L::opp(i)];
-            if (live) S::template store<i, L>(B + (size_t)i * g.plane + node, c, a.sp);
+            if (live) S::template store<i, L>(B + (size_t)i * g.ds + pnode, c, a.sp);
         });
         mp = p; mu = u; mv = v; mw = w;
     } else {
@@ -152,7 +155,7 @@ __global__ void __launch_bounds__(512)
             const float feq = (float)L::w(i) * __builtin_fmaf(eu, __builtin_fmaf(4.5f, eu, 3.0f), base);
             float c = __builtin_fmaf(om, feq - gv[i], gv[i]);
             if constexpr (SFRAC) c = __builtin_fmaf(sf, gv[L::opp(i)] - c, c);
-            if (live) S::template store<i, L>(B + (size_t)i * g.plane + node, c, a.sp);
+            if (live) S::template store<i, L>(B + (size_t)i * g.ds + pnode, c, a.sp);
         });
         mp = (double)s * (1.0 / 3.0); mu = u; mv = v; mw = w;
     }
@@ -229,7 +232,8 @@ __global__ void __launch_bounds__(512) k_decode(const typename S::T *__restrict_
     if (x >= g.nx) return;
     const size_t node = (size_t)x + (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
     const size_t phys = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
-    out[phys] = flags[node] == FL_SOLID ? 0.0 : (double)S::decode(A[node], w, off);
+    const size_t e = (size_t)x + g.rs * ((size_t)y + (size_t)g.ny * z);   // A: this direction's element 0
+    out[phys] = flags[node] == FL_SOLID ? 0.0 : (double)S::decode(A[e], w, off);
 }
 
 template <class S>
@@ -239,7 +243,8 @@ __global__ void __launch_bounds__(512) k_encode(typename S::T *__restrict__ A, c
     if (x >= g.nx) return;
     const size_t node = (size_t)x + (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
     const size_t phys = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
-    A[node] = S::encode(flags[node] == FL_SOLID ? 0.0 : in[phys], winv, offw);
+    const size_t e = (size_t)x + g.rs * ((size_t)y + (size_t)g.ny * z);
+    A[e] = S::encode(flags[node] == FL_SOLID ? 0.0 : in[phys], winv, offw);
 }
 
 __global__ void __launch_bounds__(512) k_macros_extract(const double *__restrict__ mac, const uint8_t *__restrict__ flags,
@@ -367,7 +372,7 @@ hipError_t plain_decode(int Q, int prec, const void *A, double *out, const uint8
         using S = decltype(st);
         using T = typename S::T;
         hipLaunchKernelGGL(k_decode<S>, grid_of(a.g, sh), dim3(sh.bx), 0, s,
-                           (const T *)A + (size_t)dir * a.g.plane, out, flags, (float)w, a.sp.off[dir], a.g);
+                           (const T *)A + (size_t)dir * a.g.ds, out, flags, (float)w, a.sp.off[dir], a.g);
         return hipGetLastError();
     });
 }
@@ -379,7 +384,7 @@ hipError_t plain_encode(int Q, int prec, void *A, const double *in, const uint8_
     return dispatch_store(prec, [&](auto st) {
         using S = decltype(st);
         using T = typename S::T;
-        hipLaunchKernelGGL(k_encode<S>, grid_of(a.g, sh), dim3(sh.bx), 0, s, (T *)A + (size_t)dir * a.g.plane,
+        hipLaunchKernelGGL(k_encode<S>, grid_of(a.g, sh), dim3(sh.bx), 0, s, (T *)A + (size_t)dir * a.g.ds,
                            in, flags, (float)(1.0 / w), a.sp.offw[dir], a.g);
         return hipGetLastError();
     });

@@ -285,16 +285,25 @@ def _fast_vs_per_node(pkg, monkeypatch, prec, V, Q, shape, kw, store_uref):
     assert np.array_equal(Fa, Fb)
 
 
-@pytest.mark.parametrize("prec,env,values", [("fp16s", "LBM_UNALIGNED_PULL", (None, "1")),
-                                             ("bf16s", "LBM_UNALIGNED_PULL", (None, "1")),
-                                             ("fp32", "LBM_FAST_BLOCKS_PER_CU", ("3", "0"))])
-def test_launch_variants_give_identical_results(pkg, monkeypatch, prec, env, values):
+PLANES = ("LBM_DIRECTION_PLANES", (None, "1"))
+
+
+@pytest.mark.parametrize("prec,env,values,periodic", [
+    ("fp16s", "LBM_ALIGNED_PULL", (None, "1"), False),
+    ("bf16s", "LBM_ALIGNED_PULL", (None, "1"), False),
+    ("fp16s", "LBM_ALIGNED_PULL", (None, "1"), True),
+    ("fp32", "LBM_FAST_BLOCKS_PER_CU", ("3", "0"), False),
+    ("fp16s", "LBM_FAST_BLOCKS_PER_CU", ("4", "0"), False),
+    ("fp64", *PLANES, False), ("fp32", *PLANES, False), ("fp16s", *PLANES, False), ("fp16s", *PLANES, True)])
+def test_launch_variants_give_identical_results(pkg, monkeypatch, prec, env, values, periodic):
     """The aligned 16-bit pull (aligned loads, lane shifts, LDS hand-over at wave edges) equals
-    the unaligned loads bit for bit on a 4-wave row with a cylinder and walls; the FP32 blocks-
-    per-CU cap changes only residency."""
+    the unaligned loads bit for bit on a 4-wave row with a cylinder and walls (or periodic y
+    and z); the blocks-per-CU cap changes only residency; the populations stored one plane per
+    direction instead of row-interleaved change only addresses."""
     L = pkg.lbm
     g = L.make_config(27, 512, 12, 10, tau0=0.05, u0=0.05, precision=L.PRECISIONS[prec], body=L.BODY_ZCYL,
-                      center=(100.0, 5.5, 0.0), r2=9.0, perturb=1e-3, force_interval=4)
+                      center=(100.0, 5.5, 0.0), r2=9.0, perturb=1e-3, force_interval=4,
+                      periodic=(False, periodic, periodic))
     out = []
     for v in values:
         if v is None:
