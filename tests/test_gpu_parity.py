@@ -113,6 +113,9 @@ REF_CASES = {
     "ref2d_disk": (9, 80, 36, 1, dict(center=(22.0, 17.5, 0.0), r2=20.0), 0.1, 0.036, 60),
     "ref3d_default_sphere": (27, 40, 16, 14, None, 0.1, 0.024, 30),
     "ref3d_sphere": (27, 36, 18, 16, dict(center=(10.0, 8.0, 7.0), r2=9.0), 0.1, 0.024, 30),
+    # population buffers above 256 MB (359 MB / 303 MB): the fused kernel's non-temporal stores
+    "ref3d_large_sphere": (27, 128, 96, 96, dict(center=(30.0, 47.5, 47.5), r2=64.0), 0.1, 0.024, 3),
+    "ref2d_large_disk": (9, 2048, 2048, 1, dict(center=(500.0, 1023.5, 0.0), r2=400.0), 0.1, 0.036, 3),
 }
 
 
