@@ -114,6 +114,11 @@ hipError_t ref_mem(const RefGeo &r, const double *f, const uint8_t *usr, double 
 // fpre (may be null): receives the pulled populations, i.e. the reference's f after Update
 hipError_t ref_fused(const RefGeo &r, const double *f, double *ft, double *mac, const uint8_t *usr,
                      const double *prof, double *fpre, hipStream_t s);
+// the staged step's pull (ft = stream1 of f at the interior nodes) and its Update + next
+// Collision in place on f (fpre as for ref_fused)
+hipError_t ref_stream(const RefGeo &r, const double *f, double *ft, hipStream_t s);
+hipError_t ref_update(const RefGeo &r, double *f, double *mac, const uint8_t *usr, const double *prof, double *fpre,
+                      hipStream_t s);
 int ref_mem_blocks(const RefGeo &r);
 
 // ---- adjoint (plain mode) ----
@@ -184,6 +189,10 @@ struct lbm_ctx {
 
     bool initialized = false;
     long step = 0;
+    // staged step (reference mode): 0 between steps (rf holds the next step's collided
+    // populations, rfpre the populations after Update), 1 after lbm_collide, 2 after
+    // lbm_boundary, 3 after lbm_stream
+    int stage = 0;
 
     SnapshotJob *snap = nullptr;
 };

@@ -631,6 +631,7 @@ extern "C" int lbm_initialize(lbm_ctx *c) {
     }
     c->initialized = true;
     c->step = 0;
+    c->stage = 0;
     c->nrec = c->nrec_checked = 0;
     return LBM_OK;
 }
@@ -662,6 +663,8 @@ StepBufs step_bufs(const lbm_ctx *c, const void *A, void *B) {
 
 // evs: null, or 2n events recorded around each step's bulk kernel (plain mode)
 static int step_n(lbm_ctx *c, int n, hipEvent_t *evs = nullptr) {
+    if (c->stage != 0)
+        return set_err(c, LBM_ERR_STATE, "a staged step is in progress: finish it with lbm_boundary, lbm_stream, lbm_update");
     const int fi = c->cfg.force_interval;
     const int nforce = fi > 0 ? (int)((c->step + n) / fi - c->step / fi) : 0;
     int rc = grow_hist(c, c->nrec + nforce + 1);
@@ -701,6 +704,63 @@ extern "C" int lbm_step(lbm_ctx *c, int n) {
     if (n < 0) return set_err(c, LBM_ERR_INVALID, "negative step count");
     if (n == 0) return LBM_OK;
     return step_n(c, n);
+}
+
+// ---- staged step (reference mode) ----
+// The fused step leaves rf holding the next step's collided populations (the reference's first
+// Collision acts on f = feq and changes nothing, and every later one is applied right after
+// Update), so lbm_collide only opens the step; lbm_boundary, lbm_stream and lbm_update then run
+// the reference's Streaming() in two parts and its Update() (followed by the next Collision, in
+// place), leaving the same state as lbm_step(ctx, 1).
+static int staged(lbm_ctx *c, int want, const char *name) {
+    if (!c) return LBM_ERR_INVALID;
+    if (!c->initialized) return set_err(c, LBM_ERR_STATE, std::string(name) + " before lbm_initialize");
+    if (!ref_mode(c->cfg.bc_mode))
+        return set_err(c, LBM_ERR_INVALID, std::string(name) + ": staged calls need a reference boundary mode "
+                                                               "(the plain mode fuses the stages into one pass)");
+    static const char *const order[4] = {"lbm_update (or nothing)", "lbm_collide", "lbm_boundary", "lbm_stream"};
+    if (c->stage != want)
+        return set_err(c, LBM_ERR_STATE, std::string(name) + " must follow " + order[want]);
+    return LBM_OK;
+}
+
+extern "C" int lbm_collide(lbm_ctx *c) {
+    int rc = staged(c, 0, "lbm_collide");
+    if (rc != LBM_OK) return rc;
+    rc = grow_hist(c, c->nrec + 2);
+    if (rc != LBM_OK) return rc;
+    c->step++;
+    c->stage = 1;
+    return LBM_OK;
+}
+
+extern "C" int lbm_boundary(lbm_ctx *c) {
+    const int rc = staged(c, 1, "lbm_boundary");
+    if (rc != LBM_OK) return rc;
+    const int fi = c->cfg.force_interval;
+    HIP_TRY(c, ref_fills(c->rg, c->rf, c->rft, c->stream));
+    HIP_TRY(c, ref_usrbc(c->rg, c->rf, c->rusr, c->stream));
+    if (fi > 0 && c->step % fi == 0)
+        HIP_TRY(c, ref_mem(c->rg, c->rf, c->rusr, c->partials, c->nslots, c->hist, c->nrec++, c->step, c->stream));
+    c->stage = 2;
+    return LBM_OK;
+}
+
+extern "C" int lbm_stream(lbm_ctx *c) {
+    const int rc = staged(c, 2, "lbm_stream");
+    if (rc != LBM_OK) return rc;
+    HIP_TRY(c, ref_stream(c->rg, c->rf, c->rft, c->stream));
+    std::swap(c->rf, c->rft);   // stream2: the ghost slots of both buffers are equal
+    c->stage = 3;
+    return LBM_OK;
+}
+
+extern "C" int lbm_update(lbm_ctx *c) {
+    const int rc = staged(c, 3, "lbm_update");
+    if (rc != LBM_OK) return rc;
+    HIP_TRY(c, ref_update(c->rg, c->rf, c->rmac, c->rusr, c->rprof, c->rfpre, c->stream));
+    c->stage = 0;
+    return LBM_OK;
 }
 
 extern "C" int lbm_step_timed(lbm_ctx *c, int n, double *ms) {
@@ -825,9 +885,12 @@ extern "C" int lbm_get_populations(lbm_ctx *c, double *f) {
     if (!c || !f) return set_err(c, LBM_ERR_INVALID, "null argument");
     if (!c->initialized) return set_err(c, LBM_ERR_STATE, "lbm_get_populations before lbm_initialize");
     if (ref_mode(c->cfg.bc_mode)) {
+        // between steps the reference's f is the state after Update; inside a staged step, the
+        // state after the last stage
+        const double *src = c->stage == 0 ? c->rfpre : c->rf;
         std::vector<double> pad(c->rg.plane);
         for (int i = 0; i < c->Q; i++) {
-            HIP_TRY(c, hipMemcpyAsync(pad.data(), c->rfpre + (size_t)i * c->rg.plane, sizeof(double) * c->rg.plane,
+            HIP_TRY(c, hipMemcpyAsync(pad.data(), src + (size_t)i * c->rg.plane, sizeof(double) * c->rg.plane,
                                       hipMemcpyDeviceToHost, c->stream));
             HIP_TRY(c, hipStreamSynchronize(c->stream));
             ref_strip(c, pad, f + (size_t)i * c->N);

@@ -235,28 +235,22 @@ __global__ void __launch_bounds__(256) k_ref_mem(RefGeo r, const double *__restr
     }
 }
 
-// stream1 + stream2 + Update (with its inlet/outlet overrides) + the next Collision:
-// 3D MPI+KOKKOS+MPIIO/lbm.cpp:257-272,359-417 ; MPI Kokkos/lbm.cpp:127-139,408-457
-// NT: non-temporal stores of ft (buffers larger than the last-level cache: the next step
-// reads them from HBM anyway); small buffers keep regular stores and stay cached.
+// Update (the moments with the reference's inlet/outlet overrides) and the next Collision of
+// one interior node from its streamed populations fv: 3D MPI+KOKKOS+MPIIO/lbm.cpp:374-417 then
+// :257-272 ; MPI Kokkos/lbm.cpp:420-457 then :127-139.  The collided values go to out (the other
+// buffer, or f itself in place); fpre, when set, receives fv (the populations after Update).
+// NT: non-temporal stores of the collided values (buffers larger than the last-level cache:
+// the next step reads them from HBM anyway); small buffers keep regular stores and stay cached.
 template <int Q, bool NT>
-__global__ void __launch_bounds__(256) k_ref_fused(RefGeo r, const double *__restrict__ f, double *__restrict__ ft,
+__device__ __forceinline__ void ref_update_collide(const RefGeo &r, int i, int j, int k, size_t n, const double *fv,
                                                    double *__restrict__ mac, const uint8_t *__restrict__ usr,
-                                                   const double *__restrict__ prof, double *__restrict__ fpre) {
+                                                   const double *__restrict__ prof, double *__restrict__ fpre,
+                                                   double *out) {
     using L = Lat<Q>;
-    // interior rows only (the grid covers them), lanes from the row start so that every wave
-    // stays aligned with the row pitch; ghost slots are equal in both buffers (copy2)
     const int g = r.g;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y + g;
-    const int k = blockIdx.z + (L::D == 3 ? g : 0);
-    if (i < g || i >= r.lx - g) return;
-    const size_t n = (size_t)i + (size_t)r.px * ((size_t)j + (size_t)r.ly * k);
-    double fv[Q];
     double p = 0.0, u = 0.0, v = 0.0, w = 0.0;
     static_for<0, Q>([&](auto I) {
         constexpr int ii = I;
-        const long src = (long)n - (L::ex(ii) + (long)r.px * (L::ey(ii) + (long)r.ly * L::ez(ii)));
-        fv[ii] = f[(size_t)ii * r.plane + src];
         if (fpre) fpre[(size_t)ii * r.plane + n] = fv[ii];
         p = p + fv[ii] / 3.0;
         if constexpr (L::ex(ii) > 0) u = u + fv[ii];
@@ -287,9 +281,63 @@ __global__ void __launch_bounds__(256) k_ref_fused(RefGeo r, const double *__res
         constexpr int ii = I;
         const double feq = ref_feq<Q, ii>(p, u, v, w);
         const double h = fv[ii] - (fv[ii] - feq) / (r.tau0 + 0.5);
+        if constexpr (NT) __builtin_nontemporal_store(h, out + (size_t)ii * r.plane + n);
+        else out[(size_t)ii * r.plane + n] = h;
+    });
+}
+
+// Interior rows only (the grid covers them), lanes from the row start so that every wave stays
+// aligned with the row pitch; ghost slots are equal in both buffers (copy2).  false for the
+// ghost lanes at either end of the row.
+#define REF_INTERIOR_LANE(r, i, j, k)                                                            \
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y + (r).g;              \
+    const int k = blockIdx.z + (Lat<Q>::D == 3 ? (r).g : 0);                                 \
+    if (i < (r).g || i >= (r).lx - (r).g) return;                                             \
+    const size_t n = (size_t)i + (size_t)(r).px * ((size_t)j + (size_t)(r).ly * k)
+
+// stream1 + stream2 + Update + the next Collision, one pass:
+// 3D MPI+KOKKOS+MPIIO/lbm.cpp:257-272,359-417 ; MPI Kokkos/lbm.cpp:127-139,408-457
+template <int Q, bool NT>
+__global__ void __launch_bounds__(256) k_ref_fused(RefGeo r, const double *__restrict__ f, double *__restrict__ ft,
+                                                   double *__restrict__ mac, const uint8_t *__restrict__ usr,
+                                                   const double *__restrict__ prof, double *__restrict__ fpre) {
+    using L = Lat<Q>;
+    REF_INTERIOR_LANE(r, i, j, k);
+    double fv[Q];
+    static_for<0, Q>([&](auto I) {
+        constexpr int ii = I;
+        const long src = (long)n - (L::ex(ii) + (long)r.px * (L::ey(ii) + (long)r.ly * L::ez(ii)));
+        fv[ii] = f[(size_t)ii * r.plane + src];
+    });
+    ref_update_collide<Q, NT>(r, i, j, k, n, fv, mac, usr, prof, fpre, ft);
+}
+
+// stream1 alone (the staged step): ft = f pulled along e_i at the interior nodes;
+// 3D MPI+KOKKOS+MPIIO/lbm.cpp:359-364 ; MPI Kokkos/lbm.cpp:408-412.  stream2's copy back is
+// the caller's buffer swap (the ghost slots of both buffers are equal).
+template <int Q, bool NT>
+__global__ void __launch_bounds__(256) k_ref_stream(RefGeo r, const double *__restrict__ f, double *__restrict__ ft) {
+    using L = Lat<Q>;
+    REF_INTERIOR_LANE(r, i, j, k);
+    static_for<0, Q>([&](auto I) {
+        constexpr int ii = I;
+        const long src = (long)n - (L::ex(ii) + (long)r.px * (L::ey(ii) + (long)r.ly * L::ez(ii)));
+        const double h = f[(size_t)ii * r.plane + src];
         if constexpr (NT) __builtin_nontemporal_store(h, ft + (size_t)ii * r.plane + n);
         else ft[(size_t)ii * r.plane + n] = h;
     });
+}
+
+// Update and the next Collision in place (the staged step): every lane reads its node's Q
+// populations before it writes them back, and no lane touches another node
+template <int Q, bool NT>
+__global__ void __launch_bounds__(256) k_ref_update(RefGeo r, double *f, double *__restrict__ mac,
+                                                    const uint8_t *__restrict__ usr, const double *__restrict__ prof,
+                                                    double *__restrict__ fpre) {
+    REF_INTERIOR_LANE(r, i, j, k);
+    double fv[Q];
+    static_for<0, Q>([&](auto I) { fv[I] = f[(size_t)I * r.plane + n]; });
+    ref_update_collide<Q, NT>(r, i, j, k, n, fv, mac, usr, prof, fpre, f);
 }
 
 __global__ void __launch_bounds__(256) k_ref_reduce(const double *__restrict__ partials, int n, double *__restrict__ hist,
@@ -384,17 +432,52 @@ hipError_t ref_mem(const RefGeo &r, const double *f, const uint8_t *usr, double 
     return hipGetLastError();
 }
 
+// grid of the interior-row kernels (lanes from the row start) and their store kind: measured,
+// 3D 512x128x128 (2.1 GB per buffer) 9 % faster with non-temporal stores, 2D 512x128 (4.9 MB)
+// 12 % slower; 256 MB is the size of the last-level (Infinity) cache
+static dim3 row_grid(const RefGeo &r) { return dim3((r.lx + 255) / 256, r.gly, r.D == 3 ? r.glz : 1); }
+static bool nt_stores(const RefGeo &r) { return (double)r.Q * (double)r.plane * sizeof(double) > 256.0 * (1 << 20); }
+
+// K<Q, NT> launched over the interior rows with the store kind for r's buffers
+template <template <int, bool> class K, class... Args>
+static hipError_t launch_rows(const RefGeo &r, hipStream_t s, Args... args) {
+    const dim3 grid = row_grid(r);
+    const bool nt = nt_stores(r);
+    if (r.Q == 9 && nt) K<9, true>::launch(grid, s, r, args...);
+    else if (r.Q == 9) K<9, false>::launch(grid, s, r, args...);
+    else if (nt) K<27, true>::launch(grid, s, r, args...);
+    else K<27, false>::launch(grid, s, r, args...);
+    return hipGetLastError();
+}
+
+template <int Q, bool NT> struct FusedK {
+    template <class... A> static void launch(dim3 g, hipStream_t s, A... a) {
+        hipLaunchKernelGGL((k_ref_fused<Q, NT>), g, dim3(256), 0, s, a...);
+    }
+};
+template <int Q, bool NT> struct StreamK {
+    template <class... A> static void launch(dim3 g, hipStream_t s, A... a) {
+        hipLaunchKernelGGL((k_ref_stream<Q, NT>), g, dim3(256), 0, s, a...);
+    }
+};
+template <int Q, bool NT> struct UpdateK {
+    template <class... A> static void launch(dim3 g, hipStream_t s, A... a) {
+        hipLaunchKernelGGL((k_ref_update<Q, NT>), g, dim3(256), 0, s, a...);
+    }
+};
+
 hipError_t ref_fused(const RefGeo &r, const double *f, double *ft, double *mac, const uint8_t *usr,
                      const double *prof, double *fpre, hipStream_t s) {
-    const dim3 grid((r.lx + 255) / 256, r.gly, r.D == 3 ? r.glz : 1);
-    // measured: 3D 512x128x128 (2.1 GB per buffer) 9 % faster with non-temporal stores, 2D
-    // 512x128 (4.9 MB) 12 % slower; 256 MB is the size of the last-level (Infinity) cache
-    const bool nt = (double)r.Q * (double)r.plane * sizeof(double) > 256.0 * (1 << 20);
-    if (r.Q == 9 && nt) hipLaunchKernelGGL((k_ref_fused<9, true>), grid, dim3(256), 0, s, r, f, ft, mac, usr, prof, fpre);
-    else if (r.Q == 9) hipLaunchKernelGGL((k_ref_fused<9, false>), grid, dim3(256), 0, s, r, f, ft, mac, usr, prof, fpre);
-    else if (nt) hipLaunchKernelGGL((k_ref_fused<27, true>), grid, dim3(256), 0, s, r, f, ft, mac, usr, prof, fpre);
-    else hipLaunchKernelGGL((k_ref_fused<27, false>), grid, dim3(256), 0, s, r, f, ft, mac, usr, prof, fpre);
-    return hipGetLastError();
+    return launch_rows<FusedK>(r, s, f, ft, mac, usr, prof, fpre);
+}
+
+hipError_t ref_stream(const RefGeo &r, const double *f, double *ft, hipStream_t s) {
+    return launch_rows<StreamK>(r, s, f, ft);
+}
+
+hipError_t ref_update(const RefGeo &r, double *f, double *mac, const uint8_t *usr, const double *prof, double *fpre,
+                      hipStream_t s) {
+    return launch_rows<UpdateK>(r, s, f, mac, usr, prof, fpre);
 }
 
 }  // namespace lbm

@@ -75,6 +75,10 @@ def lib():
             "lbm_step": (I, [P, I]),
             "lbm_step_timed": (I, [P, I, C.POINTER(C.c_double)]),
             "lbm_sync": (I, [P]),
+            "lbm_collide": (I, [P]),
+            "lbm_boundary": (I, [P]),
+            "lbm_stream": (I, [P]),
+            "lbm_update": (I, [P]),
             "lbm_get_fields": (I, [P, P, P, P, P]),
             "lbm_get_populations": (I, [P, P]),
             "lbm_set_populations": (I, [P, P]),
@@ -212,6 +216,19 @@ class Solver:
 
     def sync(self):
         self._chk(lib().lbm_sync(self.h))
+
+    # one step in the reference's stages (reference boundary modes), in this order
+    def collide(self):
+        self._chk(lib().lbm_collide(self.h))
+
+    def boundary(self):
+        self._chk(lib().lbm_boundary(self.h))
+
+    def stream(self):
+        self._chk(lib().lbm_stream(self.h))
+
+    def update(self):
+        self._chk(lib().lbm_update(self.h))
 
     # data out
     def fields(self):

@@ -128,6 +128,26 @@ int lbm_step_timed(lbm_ctx *ctx, int n, double *ms);
  * storage), between events recorded just before and after its launch.  Plain mode only,
  * 1 <= n <= 100000.  A measurement entry point with no counterpart in the reference. */
 int lbm_step_kernel_timed(lbm_ctx *ctx, int n, double *ms_steps, double *ms_bulk);
+/* One time step in the reference's stages, callable one at a time and only in this order
+ * (reference boundary modes; LBM_ERR_INVALID in the plain mode, which fuses the stages;
+ * LBM_ERR_STATE out of order, and lbm_step / lbm_step_timed inside an unfinished step).
+ * Four calls leave the same state as lbm_step(ctx, 1); lbm_get_populations returns the
+ * populations as the reference's f stands after the last stage, lbm_get_fields the moments of
+ * the last Update.  Asynchronous on the context stream.
+ *   lbm_collide   LBM::Collision()  (3D MPI+KOKKOS+MPIIO/lbm.cpp:257-272; MPI Kokkos/lbm.cpp:127-139);
+ *                 opens the step: every Collision is applied right after the Update
+ *                 before it (the first acts on f = feq and changes nothing), so the
+ *                 collided populations are already in place
+ *   lbm_boundary  the ghost fills and usrbc of LBM::Streaming() (lbm.cpp:278-356; MPI Kokkos/
+ *                 lbm.cpp:315-353), with the momentum exchange on force steps (MPI Kokkos/
+ *                 lbm.cpp:356-407)
+ *   lbm_stream    stream1 + stream2 of LBM::Streaming() (lbm.cpp:359-370; MPI Kokkos/lbm.cpp:408-416)
+ *   lbm_update    LBM::Update() (lbm.cpp:374-417; MPI Kokkos/lbm.cpp:420-457), then the next
+ *                 step's Collision in place */
+int lbm_collide(lbm_ctx *ctx);
+int lbm_boundary(lbm_ctx *ctx);
+int lbm_stream(lbm_ctx *ctx);
+int lbm_update(lbm_ctx *ctx);
 /* drains the context stream and checks the forces recorded since the last check */
 int lbm_sync(lbm_ctx *ctx);
 

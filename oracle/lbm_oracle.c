@@ -456,6 +456,35 @@ void orc_ref_step(orc_ref *s, int nsteps)
     }
 }
 
+/* One stage of a step, for the staged C ABI: 0 Collision (opens the step: step count + 1),
+ * 1 the fills + usrbc part of Streaming with the momentum exchange on force steps, 2 stream1 +
+ * stream2, 3 Update.  The reference's method split: 3D MPI+KOKKOS+MPIIO/lbm.cpp:257-272
+ * (Collision), 278-370 (Streaming), 374-417 (Update); main.cpp:29-37 calls them in this order. */
+void orc_ref_stage(orc_ref *s, int stage)
+{
+    switch (stage) {
+    case 0:
+        s->step++;
+        orc_ref_collision(s);
+        break;
+    case 1:
+        orc_ref_boundary_fills(s);
+        orc_ref_usrbc(s);
+        if (s->force_interval > 0 && s->step % s->force_interval == 0) {
+            double F[3];
+            orc_ref_mem(s, F);
+            push_force(&s->frec, &s->nrec, &s->caprec, s->step, F);
+        }
+        break;
+    case 2:
+        orc_ref_stream(s);
+        break;
+    default:
+        orc_ref_update(s);
+        break;
+    }
+}
+
 void orc_ref_fields(const orc_ref *s, double *p, double *u, double *v, double *w)
 {
     const int g = s->g, gz = (s->L.D == 3) ? g : 0;

@@ -74,6 +74,9 @@ orc_ref *orc_ref_create(const orc_cfg *cfg, const int *ran /* lx*ly*lz padded, o
 void     orc_ref_destroy(orc_ref *s);
 void     orc_ref_extents(const orc_ref *s, int *lx, int *ly, int *lz, int *ghost);
 void     orc_ref_step(orc_ref *s, int nsteps);
+/* one stage of a step: 0 Collision (step count + 1), 1 fills + usrbc (+ momentum exchange on
+ * force steps), 2 stream1 + stream2, 3 Update */
+void     orc_ref_stage(orc_ref *s, int stage);
 /* fields of the last Update (interior, x fastest) -- w may be NULL in 2D */
 void     orc_ref_fields(const orc_ref *s, double *p, double *u, double *v, double *w);
 /* populations f at interior nodes, SoA [q][z][y][x] */

@@ -60,6 +60,7 @@ def lib():
         L.orc_ref_destroy.argtypes = [P]
         L.orc_ref_extents.argtypes = [P] + [C.POINTER(C.c_int)] * 4
         L.orc_ref_step.argtypes = [P, C.c_int]
+        L.orc_ref_stage.argtypes = [P, C.c_int]
         L.orc_ref_fields.argtypes = [P, P, P, P, P]
         L.orc_ref_populations.argtypes = [P, P]
         L.orc_ref_raw.restype = C.POINTER(C.c_double)
@@ -152,6 +153,10 @@ class RefState:
 
     def step(self, n=1):
         lib().orc_ref_step(self.h, n)
+
+    def stage(self, k):
+        """0 Collision, 1 fills + usrbc (+ momentum exchange), 2 stream1 + stream2, 3 Update."""
+        lib().orc_ref_stage(self.h, k)
 
     def fields(self):
         out = [np.zeros(self.shape) for _ in range(4)]

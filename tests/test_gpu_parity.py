@@ -147,6 +147,74 @@ def test_reference_mode_matches_oracle(pkg, orc, name):
     assert np.abs(Fg[:, 1:] - Fo[:, 1:]).max() <= 1e-11 * max(np.abs(Fo[:, 1:]).max(), 1e-300)
 
 
+@pytest.mark.parametrize("name,warm", [("ref2d_disk", 4), ("ref3d_sphere", 4), ("ref3d_sphere", 0)])
+def test_staged_step_matches_oracle_stages(pkg, orc, name, warm):
+    """lbm_collide / lbm_boundary / lbm_stream / lbm_update against the oracle's stages of the
+    reference step: the populations after every stage and the fields after every Update to
+    round-off (1e-13), forces to 1e-11; starting right after lbm_initialize or after fused
+    steps, and fused steps again after the staged ones."""
+    L = pkg.lbm
+    Q, nx, ny, nz, body, u0, tau0, _ = REF_CASES[name]
+    g = L.make_config(Q, nx, ny, nz, tau0=tau0, u0=u0, precision=L.FP64, bc_mode=L.BC_REFERENCE,
+                      body=L.BODY_SPHERE, force_interval=2, **body)
+    o = orc.make_cfg(Q, nx, ny, nz, tau0=tau0, u0=u0, body=orc.BODY_SPHERE, force_interval=2, **body)
+    print(f"[parity] {name} staged step after {warm} fused steps", flush=True)
+    with pkg.Solver(g) as gs:
+        gs.initialize()
+        rs = orc.RefState(o)
+        if warm:
+            gs.step(warm)
+            rs.step(warm)
+        for cycle in range(3):
+            for k, call in enumerate((gs.collide, gs.boundary, gs.stream, gs.update)):
+                call()
+                rs.stage(k)
+                a, b = gs.populations(), rs.populations()
+                assert rel_l2(a, b) <= 1e-13, (cycle, k)
+                assert np.abs(a - b).max() <= 1e-13 * max(1.0, np.abs(b).max()), (cycle, k)
+            for a, b in zip(gs.fields(), rs.fields()):
+                assert np.abs(a - b).max() <= 1e-13 * max(1.0, np.abs(b).max())
+        gs.step(3)
+        rs.step(3)
+        assert rel_l2(gs.populations(), rs.populations()) <= 1e-13
+        Fg, Fo = gs.forces(), rs.forces()
+    assert Fg.shape == Fo.shape == ((warm + 6) // 2, 4)
+    assert np.array_equal(Fg[:, 0], Fo[:, 0])
+    assert np.abs(Fg[:, 1:] - Fo[:, 1:]).max() <= 1e-11 * max(np.abs(Fo[:, 1:]).max(), 1e-300)
+
+
+def test_staged_step_call_order(pkg):
+    """Out of order -> LBM_ERR_STATE (4), lbm_step inside an unfinished step -> LBM_ERR_STATE, the
+    plain mode -> LBM_ERR_INVALID (1); the step then completes normally."""
+    L = pkg.lbm
+    g = L.make_config(9, 40, 20, tau0=0.036, u0=0.1, precision=L.FP64, bc_mode=L.BC_REFERENCE,
+                      body=L.BODY_SPHERE, center=(10.0, 9.5, 0.0), r2=9.0)
+    with pkg.Solver(g) as s:
+        s.initialize()
+        for bad in (s.boundary, s.stream, s.update):
+            with pytest.raises(L.LbmError) as e:
+                bad()
+            assert e.value.code == 4
+        s.collide()
+        with pytest.raises(L.LbmError) as e:
+            s.step(1)
+        assert e.value.code == 4
+        with pytest.raises(L.LbmError) as e:
+            s.collide()
+        assert e.value.code == 4
+        s.boundary()
+        s.stream()
+        s.update()
+        s.step(2)
+        assert np.isfinite(s.fields()[1]).all()
+    p = L.make_config(9, 40, 20, tau0=0.036, u0=0.1, precision=L.FP64, bc_mode=L.BC_PLAIN)
+    with pkg.Solver(p) as s:
+        s.initialize()
+        with pytest.raises(L.LbmError) as e:
+            s.collide()
+        assert e.value.code == 1
+
+
 def test_reference_mode_injected_ran(pkg, orc):
     """An injected ran field (in place of the Kokkos pool draw) reaches both sides identically."""
     L = pkg.lbm

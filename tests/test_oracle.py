@@ -183,6 +183,32 @@ def test_reference_update_overrides(orc, Q):
     assert F.shape == (3, 4) and np.array_equal(F[:, 0], [1, 2, 3]) and np.all(np.isfinite(F))
 
 
+@pytest.mark.parametrize("Q,variant", [(9, 0), (9, 1), (27, 0)])
+def test_reference_stages_compose_to_the_step(orc, Q, variant):
+    """The four stages of the staged C ABI (Collision; fills + usrbc + momentum exchange;
+    stream1 + stream2; Update), called in the reference main's order, reproduce orc_ref_step
+    bit for bit, forces included; a stage order with stream before the fills does not."""
+    nx, ny, nz = (30, 14, 1) if Q == 9 else (16, 10, 12)
+    kw = dict(tau0=0.03, u0=0.1, body=orc.BODY_SPHERE, center=(8.0, 5.0, 6.0), r2=4.0, force_interval=2)
+    if variant:
+        kw["variant"] = 1
+    a = orc.RefState(orc.make_cfg(Q, nx, ny, nz, **kw))
+    b = orc.RefState(orc.make_cfg(Q, nx, ny, nz, **kw))
+    a.step(5)
+    for _ in range(5):
+        for k in range(4):
+            b.stage(k)
+    assert np.array_equal(a.raw(), b.raw())
+    for x, y in zip(a.fields(), b.fields()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a.forces(), b.forces()) and a.forces().shape == (2, 4)
+    c = orc.RefState(orc.make_cfg(Q, nx, ny, nz, **kw))
+    for _ in range(5):
+        for k in (0, 2, 1, 3):
+            c.stage(k)
+    assert not np.array_equal(a.populations(), c.populations())
+
+
 # ------------------------------------------------------- plain mode vs an independent numpy
 def numpy_periodic_step(f, e, w, tau0, D):
     """Textbook pull streaming on a periodic box + the shifted incompressible BGK."""
