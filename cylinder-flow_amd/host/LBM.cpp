@@ -60,18 +60,24 @@ void LBM::Initialize() { check(lbm_initialize(ctx), "lbm_initialize"); }
 
 void LBM::Collision() {
     if (stage != 0) throw std::runtime_error("Collision() called twice without Streaming()/Update()");
+    if (opt.staged) check(lbm_collide(ctx), "lbm_collide");
     stage = 1;
 }
 
 void LBM::Streaming() {
     if (stage != 1) throw std::runtime_error("Streaming() must follow Collision()");
+    if (opt.staged) {
+        check(lbm_boundary(ctx), "lbm_boundary");
+        check(lbm_stream(ctx), "lbm_stream");
+    }
     stage = 2;
 }
 
 void LBM::Update() {
     if (stage != 2) throw std::runtime_error("Update() must follow Collision() and Streaming()");
     stage = 0;
-    check(lbm_step(ctx, 1), "lbm_step");
+    if (opt.staged) check(lbm_update(ctx), "lbm_update");
+    else check(lbm_step(ctx, 1), "lbm_step");
     if (opt.log_test_dat && cfg.force_interval > 0) {
         std::vector<double> F = forces();
         const int n = (int)F.size() / 4;

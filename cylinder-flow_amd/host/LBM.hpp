@@ -38,6 +38,9 @@ struct LBMOptions {
     bool log_test_dat = false;
     bool single_node = false;     // 2D single-node variant: LBM(sx, sy, ...) holds sx+1 by sy+1
                                   // nodes (Kokkos/lbm.hpp:26), its disk and boundary rules
+    bool staged = false;          // reference modes: Collision(), Streaming() and Update() each run
+                                  // their own stage (lbm_collide, lbm_boundary + lbm_stream,
+                                  // lbm_update) instead of one fused step in Update()
 };
 
 struct LBM {

@@ -49,21 +49,24 @@ static LBMOptions options_from(const System &s) {
     o.force_interval = std::atoi(s.option("force_interval", "-1").c_str());
     o.cd_length = std::atof(s.option("cd_length", "0").c_str());
     o.log_test_dat = s.option("test_dat", "no") == "yes";
+    o.staged = s.option("staged", "no") == "yes";
     return o;
 }
 
 int main(int argc, char **argv) {
     std::string input = "input.in";
     long cap = -1;
-    bool check = false, two_d = false, single = false;
+    bool check = false, two_d = false, single = false, staged = false;
     for (int a = 1; a < argc; a++) {
         if (!std::strcmp(argv[a], "--input") && a + 1 < argc) input = argv[++a];
         else if (!std::strcmp(argv[a], "--time") && a + 1 < argc) cap = std::atol(argv[++a]);
         else if (!std::strcmp(argv[a], "--check")) check = true;
         else if (!std::strcmp(argv[a], "--2d")) two_d = true;
         else if (!std::strcmp(argv[a], "--2d-single")) two_d = single = true;
+        else if (!std::strcmp(argv[a], "--staged")) staged = true;
         else {
-            std::fprintf(stderr, "usage: %s [--input input.in] [--time N] [--check] [--2d | --2d-single]\n", argv[0]);
+            std::fprintf(stderr, "usage: %s [--input input.in] [--time N] [--check] [--2d | --2d-single] [--staged]\n",
+                         argv[0]);
             return 2;
         }
     }
@@ -79,6 +82,7 @@ int main(int argc, char **argv) {
             o.lattice = 9;
             o.single_node = single;
             o.log_test_dat = !single;
+            o.staged = staged;
             double tau = s1.tau, rho0 = s1.rho0, u0 = s1.u0;
             LBM l1(0, s1.sx, s1.sy, 1, tau, rho0, u0, o);
             l1.Initialize();
@@ -101,6 +105,7 @@ int main(int argc, char **argv) {
         s1.Monitor();
         if (check) return 0;
         LBMOptions o = options_from(s1);
+        if (staged) o.staged = true;
         LBM l1(0, s1.sx, s1.sy, s1.sz, s1.tau, s1.rho0, s1.u0, o);
         l1.Initialize();
         l1.MPIoutput(0);

@@ -149,3 +149,25 @@ def test_driver_2d_single_node_variant(pkg, tmp_path):
     for k, f in enumerate((u, v, p)):
         assert np.abs(a[:, 2 + k] - f.ravel()).max() <= 1e-8
     assert not (tmp_path / "test.dat").exists() and not (tmp_path / "output00.dat").exists()
+
+
+@pytest.mark.parametrize("mode", ["3d", "2d"])
+def test_driver_staged_equals_fused(tmp_path, mode):
+    """LBMOptions::staged (`staged = yes` in input.in, or --staged): Collision(), Streaming() and
+    Update() each run their own stage through lbm_collide / lbm_boundary + lbm_stream /
+    lbm_update; the snapshots and the test.dat drag record equal the fused run's byte for byte."""
+    outs = []
+    for staged in (False, True):
+        d = tmp_path / ("staged" if staged else "fused")
+        d.mkdir()
+        if mode == "3d":
+            (d / "input.in").write_text("1.0 2.0 10\n0.1 100 10\n48 24 24\n=============\nrho R Re\n"
+                                        + ("staged = yes\n" if staged else ""))
+            run_driver(["--time", "120"], d)
+            names = ["output0.plt", "output1.plt"]
+        else:
+            run_driver(["--2d", "--time", "120"] + (["--staged"] if staged else []), d)
+            names = ["output00.dat", "output10.dat", "test.dat"]
+        outs.append([(d / n).read_bytes() for n in names])
+    for a, b in zip(*outs):
+        assert len(a) > 0 and a == b
