@@ -136,15 +136,15 @@ __device__ __forceinline__ void store_pack(char *p, const Pack<T, V> &r) {
 // of a row's populations (dst) and of its nominal source row, x shift excluded (src):
 //   dst[i] = i * ds * sizeof(T),   src[i] = dst[i] - (e_iy + ny e_iz) * rs * sizeof(T)
 // computed on the host per launch, so a block addresses direction i with one scalar add.
-struct PlaneTab {
+struct DirTab {
     int64_t dst[27];
     int64_t src[27];
 };
 
 template <int Q, class T>
-inline PlaneTab plane_tab(const Geo &g) {
+inline DirTab dir_tab(const Geo &g) {
     using L = Lat<Q>;
-    PlaneTab t{};
+    DirTab t{};
     for (int i = 0; i < Q; i++) {
         t.dst[i] = (int64_t)i * (int64_t)g.ds * (int64_t)sizeof(T);
         t.src[i] = t.dst[i] - ((int64_t)L::ey(i) + (int64_t)g.ny * L::ez(i)) * (int64_t)g.rs * (int64_t)sizeof(T);
@@ -162,7 +162,7 @@ inline PlaneTab plane_tab(const Geo &g) {
 // dword, which the caller's clamp then shifts as on the unaligned path.  Each request covers
 // two 128-byte lines instead of three.
 template <int Q, class T>
-__device__ __forceinline__ void aligned_pull(const char *Arow, const PlaneTab &tab, uint32_t b0, bool first,
+__device__ __forceinline__ void aligned_pull(const char *Arow, const DirTab &tab, uint32_t b0, bool first,
                                              bool last, Pack<T, 2> *r) {
     using L = Lat<Q>;
     constexpr int NS = Q == 27 ? 9 : 3;   // directions per sign of e_x
@@ -203,7 +203,7 @@ __device__ __forceinline__ void aligned_pull(const char *Arow, const PlaneTab &t
 template <int Q, class S, int V, bool MACROS, bool ALIGNED>
 __device__ __forceinline__ void packed_nodes(const typename S::T *__restrict__ A, char *__restrict__ Brow,
                                              const double *__restrict__ uin, double *__restrict__ macros,
-                                             const FusedArgs &a, const PlaneTab &tab, int y, int z, int x0,
+                                             const FusedArgs &a, const DirTab &tab, int y, int z, int x0,
                                              size_t rowoff, size_t poprow) {
     using L = Lat<Q>;
     using T = typename S::T;
@@ -301,7 +301,7 @@ __device__ __forceinline__ void packed_nodes(const typename S::T *__restrict__ A
 template <int Q, class S, int V, bool MACROS, bool ALIGNED>
 __global__ void __launch_bounds__(256)
     k_fast_v(const typename S::T *__restrict__ A, typename S::T *__restrict__ B, const double *__restrict__ uin,
-             double *__restrict__ macros, const FusedArgs a, const PlaneTab tab) {
+             double *__restrict__ macros, const FusedArgs a, const DirTab tab) {
     const Geo &g = a.g;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;   // lane index within the x row
     if (t * V >= g.nxp) return;
@@ -386,7 +386,7 @@ __global__ void __launch_bounds__(256)
     k_fix_v(const typename S::T *__restrict__ A, typename S::T *__restrict__ B, const uint32_t *__restrict__ list,
             const uint32_t *__restrict__ links, int nlist, const int *__restrict__ slot,
             const float *__restrict__ sfrac, const double *__restrict__ uin, double *__restrict__ partials,
-            double *__restrict__ macros, const FusedArgs a, const PlaneTab tab) {
+            double *__restrict__ macros, const FusedArgs a, const DirTab tab) {
     using L = Lat<Q>;
     using T = typename S::T;
     const Geo &g = a.g;
@@ -431,7 +431,7 @@ template <int Q, class S, int V, bool FORCE, bool MACROS>
 static hipError_t fast_launch(const StepBufs &b, const FusedArgs &a, hipStream_t s) {
     using T = typename S::T;
     const Shape sh = lane_shape(a.g, V);
-    const PlaneTab tab = plane_tab<Q, T>(a.g);
+    const DirTab tab = dir_tab<Q, T>(a.g);
     hipError_t e;
     if (b.nlink > 0 && !FORCE && b.npair > 0) {
         hipLaunchKernelGGL((k_link_pairs<T>), dim3((unsigned)((b.npair + 255) / 256)), dim3(256), 0, s, (T *)b.A,

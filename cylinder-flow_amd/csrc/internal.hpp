@@ -87,7 +87,7 @@ hipError_t fast_step_bf16(int Q, int V, const StepBufs &b, bool force, bool mac,
 int force_chunks(int nslots);
 hipError_t force_reduce(const double *partials, int nslots, double *chunks, double *hist, int rec, long step,
                         hipStream_t s);
-// FP64 <-> storage conversion of one direction plane (physical nodes, x fastest)
+// FP64 <-> storage conversion of one direction's populations (physical nodes, x fastest)
 hipError_t plain_decode(int Q, int prec, const void *A, double *out, const uint8_t *flags, int dir,
                         const FusedArgs &a, hipStream_t s);
 hipError_t plain_encode(int Q, int prec, void *A, const double *in, const uint8_t *flags, int dir,

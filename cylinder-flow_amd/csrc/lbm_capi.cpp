@@ -137,7 +137,7 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_ctx **out) {
     if (cfg->force_interval < 0) return set_err(nullptr, LBM_ERR_INVALID, "force_interval must be >= 0");
     const int nxp = (cfg->nx + 63) / 64 * 64;
     const size_t plane = (size_t)nxp * cfg->ny * nz;
-    if (plane >= (size_t)1 << 32) return set_err(nullptr, LBM_ERR_INVALID, "domain too large: a direction plane must stay below 2^32 nodes");
+    if (plane >= (size_t)1 << 32) return set_err(nullptr, LBM_ERR_INVALID, "domain too large: the padded node count nxp*ny*nz must stay below 2^32");
 
     lbm_ctx *c = new lbm_ctx();
     c->cfg = *cfg;
