@@ -102,6 +102,7 @@ struct RefGeo {
     int lx, ly, lz;              // padded extents
     int px;                      // row pitch in elements (>= lx; 3D rows start 256-byte aligned)
     size_t plane;                // px*ly*lz
+    size_t ds, rs;               // population element (ii, i, j, k) = ii*ds + i + rs*(j + ly*k)
     double tau0, u0;
     int variant;                 // D2Q9: 0 the MPI variant, 1 the single-node variant
 };

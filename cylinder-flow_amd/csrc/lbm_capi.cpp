@@ -268,6 +268,13 @@ static void ref_extents(const lbm_ctx *c, RefGeo &r) {
     // one column past a row end into the next row exactly as the reference's flat index does.
     r.px = c->D == 3 ? (r.lx + 31) / 32 * 32 : r.lx;
     r.plane = (size_t)r.px * r.ly * r.lz;
+    // populations: 3D row-interleaved as in the plain path (every 3D access stays inside its
+    // row: i +- 2 lies in [0, lx) at every node a fill, usrbc or pull touches), 2D one plane
+    // per direction (the flat-index wrap above needs consecutive rows of one direction);
+    // LBM_DIRECTION_PLANES selects planes in 3D too
+    const bool planes = c->D == 2 || std::getenv("LBM_DIRECTION_PLANES") != nullptr;
+    r.ds = planes ? r.plane : (size_t)r.px;
+    r.rs = planes ? (size_t)r.px : (size_t)r.Q * r.px;
     r.tau0 = c->cfg.tau0;
     r.u0 = c->cfg.u0;
     r.variant = c->cfg.bc_mode == LBM_BC_REFERENCE_SN ? 1 : 0;
@@ -888,10 +895,12 @@ extern "C" int lbm_get_populations(lbm_ctx *c, double *f) {
         // between steps the reference's f is the state after Update; inside a staged step, the
         // state after the last stage
         const double *src = c->stage == 0 ? c->rfpre : c->rf;
-        std::vector<double> pad(c->rg.plane);
+        const RefGeo &r = c->rg;
+        std::vector<double> pad(r.plane);   // one direction in the node layout
         for (int i = 0; i < c->Q; i++) {
-            HIP_TRY(c, hipMemcpyAsync(pad.data(), src + (size_t)i * c->rg.plane, sizeof(double) * c->rg.plane,
-                                      hipMemcpyDeviceToHost, c->stream));
+            // rows of direction i: px elements every rs elements from element i*ds
+            HIP_TRY(c, hipMemcpy2DAsync(pad.data(), sizeof(double) * r.px, src + (size_t)i * r.ds, sizeof(double) * r.rs,
+                                        sizeof(double) * r.px, (size_t)r.ly * r.lz, hipMemcpyDeviceToHost, c->stream));
             HIP_TRY(c, hipStreamSynchronize(c->stream));
             ref_strip(c, pad, f + (size_t)i * c->N);
         }

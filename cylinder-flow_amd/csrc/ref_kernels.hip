@@ -1,16 +1,20 @@
 // ref_kernels.hip -- the reference's own boundary rules on gfx950 (FP64 parity path).
 //
-// Data: SoA over the ghost-padded box of the reference (ghost 3 in 3D, 2 in 2D;
-// 3D MPI+KOKKOS+MPIIO/lbm.hpp:20-22,153-163, MPI Kokkos/lbm.hpp:19,79-85):
-//   f[ii*plane + i + lx*(j + ly*k)],  plane = lx*ly*lz.
+// Data: the ghost-padded box of the reference (ghost 3 in 3D, 2 in 2D;
+// 3D MPI+KOKKOS+MPIIO/lbm.hpp:20-22,153-163, MPI Kokkos/lbm.hpp:19,79-85), rows of px >= lx
+// elements (3D: a multiple of 32, so rows start 256-byte aligned; 2D: px = lx, which keeps the
+// reference's flat-index arithmetic at the row ends).  Node arrays: a[i + px*(j + ly*k)].
+// Populations: f[RIDX(ii, i, j, k)] = f[ii*ds + i + rs*(j + ly*k)], in 3D row-interleaved
+// (ds = px, rs = Q*px: the Q direction rows of a row side by side), in 2D one plane per
+// direction (ds = px*ly, rs = px).
 // One step is
 //   fills (the reference's six wall/inlet/outlet kernels in their order, in place)
 //   -> usrbc (in place) -> [momentum exchange] -> fused pull + Update + Collision into ft
 //   -> swap f/ft.
 // Fusing stream1/stream2/Update/Collision is exact: the first reference Collision acts on
 // f = feq(macros) and returns f unchanged, so Collision_{n+1} can run right after
-// Update_n inside the pull kernel (3D MPI+KOKKOS+MPIIO/main.cpp:29-37).  Ghost slots are
-// copied f -> ft by the same kernel, so ghost values persist across steps exactly as in
+// Update_n inside the pull kernel (3D MPI+KOKKOS+MPIIO/main.cpp:29-37).  The fills write
+// every ghost slot into both buffers, so ghost values persist across the swaps exactly as in
 // the reference's single f array -- including the slots no fill ever refreshes.
 #include "internal.hpp"
 #include "lattice.hpp"
@@ -36,6 +40,12 @@ __device__ __forceinline__ double ref_feq(double p, double u, double v, double w
     const double eu2 = edu * edu;
     return L::w(I) * p * 3.0 + L::w(I) * (3.0 * edu + 4.5 * eu2 - 1.5 * udu);
 }
+
+// element of population ii at (i, j, k): ii*ds + i + rs*(j + ly*k) (one plane per direction:
+// ds = plane, rs = px; row-interleaved: ds = px, rs = Q*px).  The node arrays (usr, moments)
+// stay at i + px*(j + ly*k).
+#define RIDX(r, ii, i, j, k) \
+    ((size_t)(ii) * (r).ds + (size_t)((long)(i) + (long)(r).rs * ((long)(j) + (long)(r).ly * (long)(k))))
 
 // u0*4*c*(n-1-c)/(n-1)^2: 3D MPI+KOKKOS+MPIIO/lbm.cpp:239,405 ; MPI Kokkos/lbm.cpp:109,446
 __device__ __forceinline__ double ref_profile(double u0, int c, int n) {
@@ -83,8 +93,8 @@ __global__ void __launch_bounds__(256) k_ref_init(RefGeo r, RefBody b, double *_
     mac[3 * r.plane + n] = 0.0;
     static_for<0, Q>([&](auto I) {
         const double v = ref_feq<Q, I>(0.0, u, 0.0, 0.0);
-        f[(size_t)I * r.plane + n] = v;
-        ft[(size_t)I * r.plane + n] = v;   // the ghosts of both buffers start equal (interior: overwritten)
+        f[RIDX(r, I, i, j, k)] = v;
+        ft[RIDX(r, I, i, j, k)] = v;   // the ghosts of both buffers start equal (interior: overwritten)
     });
 }
 
@@ -97,7 +107,6 @@ __device__ __forceinline__ void copy2(double *__restrict__ f, double *__restrict
     ft[dst] = v;
 }
 
-#define RIDX(r, ii, i, j, k) ((size_t)(ii) * (r).plane + (size_t)((long)(i) + (long)(r).px * ((long)(j) + (long)(r).ly * (long)(k))))
 
 // 3D fills, one kernel per face: 3D MPI+KOKKOS+MPIIO/lbm.cpp:278-347.
 // face: 0 front (y lo), 1 back (y hi), 2 bottom (z lo), 3 top (z hi), 4 inlet, 5 outlet
@@ -178,8 +187,7 @@ __global__ void __launch_bounds__(256) k_ref_usrbc(RefGeo r, double *__restrict_
         constexpr int ii = I;
         const long nb = n + L::ex(ii) + (long)r.px * (L::ey(ii) + (long)r.ly * L::ez(ii));
         if (usr[nb] == 1) {
-            const long n2 = n + 2 * (L::ex(ii) + (long)r.px * (L::ey(ii) + (long)r.ly * L::ez(ii)));
-            f[(size_t)ii * r.plane + n] = f[(size_t)L::opp(ii) * r.plane + n2];
+            f[RIDX(r, ii, i, j, k)] = f[RIDX(r, L::opp(ii), i + 2 * L::ex(ii), j + 2 * L::ey(ii), k + 2 * L::ez(ii))];
         }
     });
 }
@@ -200,7 +208,7 @@ __global__ void __launch_bounds__(256) k_ref_mem(RefGeo r, const double *__restr
                 constexpr int ii = I;
                 const long nb = n + L::ex(ii) + (long)r.px * (L::ey(ii) + (long)r.ly * L::ez(ii));
                 if (usr[nb] == 0) {
-                    const double m = f[(size_t)ii * r.plane + n] + f[(size_t)L::opp(ii) * r.plane + nb];
+                    const double m = f[RIDX(r, ii, i, j, k)] + f[RIDX(r, L::opp(ii), i + L::ex(ii), j + L::ey(ii), k + L::ez(ii))];
                     fx += m * (double)L::ex(ii);
                     fy += m * (double)L::ey(ii);
                     fz += m * (double)L::ez(ii);
@@ -242,7 +250,8 @@ __global__ void __launch_bounds__(256) k_ref_mem(RefGeo r, const double *__restr
 // NT: non-temporal stores of the collided values (buffers larger than the last-level cache:
 // the next step reads them from HBM anyway); small buffers keep regular stores and stay cached.
 template <int Q, bool NT>
-__device__ __forceinline__ void ref_update_collide(const RefGeo &r, int i, int j, int k, size_t n, const double *fv,
+__device__ __forceinline__ void ref_update_collide(const RefGeo &r, int i, int j, int k, size_t n, size_t pe,
+                                                   const double *fv,
                                                    double *__restrict__ mac, const uint8_t *__restrict__ usr,
                                                    const double *__restrict__ prof, double *__restrict__ fpre,
                                                    double *out) {
@@ -251,7 +260,7 @@ __device__ __forceinline__ void ref_update_collide(const RefGeo &r, int i, int j
     double p = 0.0, u = 0.0, v = 0.0, w = 0.0;
     static_for<0, Q>([&](auto I) {
         constexpr int ii = I;
-        if (fpre) fpre[(size_t)ii * r.plane + n] = fv[ii];
+        if (fpre) fpre[(size_t)ii * r.ds + pe] = fv[ii];
         p = p + fv[ii] / 3.0;
         if constexpr (L::ex(ii) > 0) u = u + fv[ii];
         else if constexpr (L::ex(ii) < 0) u = u - fv[ii];
@@ -281,8 +290,8 @@ __device__ __forceinline__ void ref_update_collide(const RefGeo &r, int i, int j
         constexpr int ii = I;
         const double feq = ref_feq<Q, ii>(p, u, v, w);
         const double h = fv[ii] - (fv[ii] - feq) / (r.tau0 + 0.5);
-        if constexpr (NT) __builtin_nontemporal_store(h, out + (size_t)ii * r.plane + n);
-        else out[(size_t)ii * r.plane + n] = h;
+        if constexpr (NT) __builtin_nontemporal_store(h, out + (size_t)ii * r.ds + pe);
+        else out[(size_t)ii * r.ds + pe] = h;
     });
 }
 
@@ -293,7 +302,8 @@ __device__ __forceinline__ void ref_update_collide(const RefGeo &r, int i, int j
     const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y + (r).g;              \
     const int k = blockIdx.z + (Lat<Q>::D == 3 ? (r).g : 0);                                 \
     if (i < (r).g || i >= (r).lx - (r).g) return;                                             \
-    const size_t n = (size_t)i + (size_t)(r).px * ((size_t)j + (size_t)(r).ly * k)
+    const size_t n = (size_t)i + (size_t)(r).px * ((size_t)j + (size_t)(r).ly * k);          \
+    const size_t pe = RIDX(r, 0, i, j, k)
 
 // stream1 + stream2 + Update + the next Collision, one pass:
 // 3D MPI+KOKKOS+MPIIO/lbm.cpp:257-272,359-417 ; MPI Kokkos/lbm.cpp:127-139,408-457
@@ -306,10 +316,9 @@ __global__ void __launch_bounds__(256) k_ref_fused(RefGeo r, const double *__res
     double fv[Q];
     static_for<0, Q>([&](auto I) {
         constexpr int ii = I;
-        const long src = (long)n - (L::ex(ii) + (long)r.px * (L::ey(ii) + (long)r.ly * L::ez(ii)));
-        fv[ii] = f[(size_t)ii * r.plane + src];
+        fv[ii] = f[RIDX(r, ii, i - L::ex(ii), j - L::ey(ii), k - L::ez(ii))];
     });
-    ref_update_collide<Q, NT>(r, i, j, k, n, fv, mac, usr, prof, fpre, ft);
+    ref_update_collide<Q, NT>(r, i, j, k, n, pe, fv, mac, usr, prof, fpre, ft);
 }
 
 // stream1 alone (the staged step): ft = f pulled along e_i at the interior nodes;
@@ -321,10 +330,9 @@ __global__ void __launch_bounds__(256) k_ref_stream(RefGeo r, const double *__re
     REF_INTERIOR_LANE(r, i, j, k);
     static_for<0, Q>([&](auto I) {
         constexpr int ii = I;
-        const long src = (long)n - (L::ex(ii) + (long)r.px * (L::ey(ii) + (long)r.ly * L::ez(ii)));
-        const double h = f[(size_t)ii * r.plane + src];
-        if constexpr (NT) __builtin_nontemporal_store(h, ft + (size_t)ii * r.plane + n);
-        else ft[(size_t)ii * r.plane + n] = h;
+        const double h = f[RIDX(r, ii, i - L::ex(ii), j - L::ey(ii), k - L::ez(ii))];
+        if constexpr (NT) __builtin_nontemporal_store(h, ft + (size_t)ii * r.ds + pe);
+        else ft[(size_t)ii * r.ds + pe] = h;
     });
 }
 
@@ -336,8 +344,8 @@ __global__ void __launch_bounds__(256) k_ref_update(RefGeo r, double *f, double 
                                                     double *__restrict__ fpre) {
     REF_INTERIOR_LANE(r, i, j, k);
     double fv[Q];
-    static_for<0, Q>([&](auto I) { fv[I] = f[(size_t)I * r.plane + n]; });
-    ref_update_collide<Q, NT>(r, i, j, k, n, fv, mac, usr, prof, fpre, f);
+    static_for<0, Q>([&](auto I) { fv[I] = f[(size_t)I * r.ds + pe]; });
+    ref_update_collide<Q, NT>(r, i, j, k, n, pe, fv, mac, usr, prof, fpre, f);
 }
 
 __global__ void __launch_bounds__(256) k_ref_reduce(const double *__restrict__ partials, int n, double *__restrict__ hist,
