@@ -85,6 +85,8 @@ def test_null_handles_are_errors_not_crashes(pkg):
     assert lib.lbm_step(None, 1) != 0
     assert lib.lbm_initialize(None) != 0
     assert lib.lbm_sync(None) != 0
+    for stage in (lib.lbm_collide, lib.lbm_boundary, lib.lbm_stream, lib.lbm_update):
+        assert stage(None) == 1          # LBM_ERR_INVALID
     lib.lbm_destroy(None)
 
 
