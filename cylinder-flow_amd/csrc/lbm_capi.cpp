@@ -173,13 +173,16 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_ctx **out) {
     // direction; LBM_ALIGNED_PULL selects aligned loads with lane shifts instead (same
     // results; faster with one plane per direction, slower with the row-interleaved layout).
     c->aligned_pull = std::getenv("LBM_ALIGNED_PULL") != nullptr;
-    // Blocks of the packed kernel resident per CU (256-lane blocks, measured on the 512x256x256
-    // cylinder with row-interleaved populations): FP32 runs fastest at 3 (3 waves per SIMD:
-    // 71.5 % of HBM peak per step against 70.8 % uncapped), FP16/BF16 at 4 (69.9 % against
-    // 67.3 % for FP16).  The cap is a dynamic LDS allocation the kernel does not use.
-    // LBM_FAST_BLOCKS_PER_CU overrides it (0: no cap).  Rows of fewer than 256 lanes (narrow
-    // domains) keep the same number of waves per CU.
-    int bpc = c->prec == LBM_FP32 ? 3 : c->prec == LBM_FP64 ? 0 : 4;
+    // Blocks of the packed kernel resident per CU (256-lane blocks, row-interleaved populations).
+    // D3Q27, measured on the 512x256x256 cylinder: FP32 fastest at 3 (3 waves per SIMD: 71.5 %
+    // of HBM peak per step against 70.8 % uncapped), FP16/BF16 at 4 (69.9 % against 67.3 % for
+    // FP16).  D2Q9, measured on an 8192x4096 disk: FP32 fastest uncapped (78.5 % against 69.3 %
+    // at 3), FP16/BF16 at 6 (73.5 % against 71.7 % uncapped and 67.4 % at 4).  The cap is a
+    // dynamic LDS allocation the kernel does not use.  LBM_FAST_BLOCKS_PER_CU overrides it
+    // (0: no cap).  Rows of fewer than 256 lanes (narrow domains) keep the same waves per CU.
+    int bpc = 0;
+    if (c->prec == LBM_FP32) bpc = Q == 27 ? 3 : 0;
+    else if (c->prec != LBM_FP64) bpc = Q == 27 ? 4 : 6;
     if (const char *env = std::getenv("LBM_FAST_BLOCKS_PER_CU")) bpc = std::max(0, std::atoi(env));
     hipError_t e = hipSuccess;
     if (bpc > 0) {
