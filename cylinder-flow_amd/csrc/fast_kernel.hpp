@@ -425,14 +425,87 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// Dense mode: when the nodes the packed path gets wrong are a large part of the domain (a
+// solid fraction over a noisy field), one pass with one node per lane replaces the copies, the
+// packed kernel and the fix-up: the per-node pull classifying its sources by the node flags,
+// the velocity inlet and collide() as in the other kernels, so the populations are those of
+// the list path bit for bit; FORCE sums the nodes' momentum exchange and partial bounce-back
+// force per block into partials[3 * block_slot[block]] (blocks without a forced node: -1).
+template <int Q, class S, bool FORCE, bool MACROS, bool SFRAC>
+__global__ void __launch_bounds__(256)
+    k_dense_v(const typename S::T *__restrict__ A, typename S::T *__restrict__ B, const uint8_t *__restrict__ flags,
+              const float *__restrict__ sfrac, const double *__restrict__ uin, const int *__restrict__ block_slot,
+              double *__restrict__ partials, double *__restrict__ macros, const FusedArgs a, const DirTab tab) {
+    using L = Lat<Q>;
+    using T = typename S::T;
+    const Geo &g = a.g;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+    const bool active = x < g.nx;
+    const uint32_t row = (uint32_t)y + (uint32_t)g.ny * (uint32_t)z;
+    // lanes past the row use the row's first node as a valid, unused address
+    const uint32_t node = (uint32_t)(active ? x : 0) + (uint32_t)g.nxp * row;
+    const uint8_t fl = active ? flags[node] : (uint8_t)FL_SOLID;
+    const bool live = fl != FL_SOLID;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    float gv[Q], c[Q], s, u, v, w;
+    pull<Q, S, FORCE>(A, flags, g, a.sp, x, y, z, node, live, fl == FL_NEAR, gv, fx, fy, fz);
+    if (!g.xper && x == 0 && live) velocity_inlet<Q, float>(gv, (float)uin[y + g.ny * z]);
+    float sf = 0.f;
+    if constexpr (SFRAC) sf = live ? sfrac[node] : 0.f;
+    collide<Q, FORCE, SFRAC>(gv, sf, a.omega, c, s, u, v, w, fx, fy, fz);
+    if (live) {
+        char *const Bn = (char *)B + pel(g, 0, (uint32_t)x, row) * sizeof(T);
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = I;
+            __builtin_nontemporal_store(S::template enc<i, L>(c[i], a.sp), (T *)(Bn + tab.dst[i]));
+        });
+    }
+    if constexpr (MACROS) {
+        if (active) {
+            double *m = macros + node;
+            m[0] = live ? (double)s * (1.0 / 3.0) : 0.0;
+            m[g.plane] = live ? (double)u : 0.0;
+            m[2 * g.plane] = live ? (double)v : 0.0;
+            m[3 * g.plane] = live ? (double)w : 0.0;
+        }
+    }
+    if constexpr (FORCE) {
+        const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        const int slot = block_slot[bid];
+        if (slot >= 0) {   // uniform over the block
+            __shared__ double red[3 * 4];
+            block_sum3<4>(fx, fy, fz, red);
+            if (threadIdx.x == 0) {
+                partials[3 * (size_t)slot] = fx;
+                partials[3 * (size_t)slot + 1] = fy;
+                partials[3 * (size_t)slot + 2] = fz;
+            }
+        }
+    }
+}
+
 // One step, three launches on s: the bounce-back copies into A's solid slots, the packed path
-// everywhere, then the listed nodes.
+// everywhere, then the listed nodes (dense mode: k_dense_v alone).
 template <int Q, class S, int V, bool FORCE, bool MACROS>
 static hipError_t fast_launch(const StepBufs &b, const FusedArgs &a, hipStream_t s) {
     using T = typename S::T;
     const Shape sh = lane_shape(a.g, V);
     const DirTab tab = dir_tab<Q, T>(a.g);
     hipError_t e;
+    if (b.dense) {
+        const Shape s1 = lane_shape(a.g, 1);
+        const dim3 grid(s1.gx, a.g.ny, a.g.nz);
+        if (b.ev_bulk[0] && (e = hipEventRecord(b.ev_bulk[0], s)) != hipSuccess) return e;
+        if (b.sfrac)
+            hipLaunchKernelGGL((k_dense_v<Q, S, FORCE, MACROS, true>), grid, dim3(s1.bx), 0, s, (const T *)b.A, (T *)b.B,
+                               b.flags, (const float *)b.sfrac, b.uin, b.block_slot, b.partials, b.macros, a, tab);
+        else
+            hipLaunchKernelGGL((k_dense_v<Q, S, FORCE, MACROS, false>), grid, dim3(s1.bx), 0, s, (const T *)b.A, (T *)b.B,
+                               b.flags, (const float *)nullptr, b.uin, b.block_slot, b.partials, b.macros, a, tab);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if (b.ev_bulk[1] && (e = hipEventRecord(b.ev_bulk[1], s)) != hipSuccess) return e;
+        return hipSuccess;
+    }
     if (b.nlink > 0 && !FORCE && b.npair > 0) {
         hipLaunchKernelGGL((k_link_pairs<T>), dim3((unsigned)((b.npair + 255) / 256)), dim3(256), 0, s, (T *)b.A,
                            b.lpair, b.npair);

@@ -73,6 +73,8 @@ struct StepBufs {
     const int *block_slot;      // force partial slot per block (FP64) or per listed node (-1: none)
     double *partials;
     double *macros;             // p, u, v, w planes (written when macros_on)
+    bool dense;                 // FP32/16-bit: one per-node pass over every node (k_dense_v)
+                                // instead of the copies, the packed pass and the fix-up
     bool aligned_pull;          // 16-bit, two nodes per lane: aligned loads + lane shifts (k_fast_v)
     unsigned fast_lds;          // dynamic LDS bytes per k_fast_v block (caps its blocks per CU; 0: none)
     hipEvent_t ev_bulk[2];      // when set: recorded just before and after the step's bulk kernel
@@ -176,7 +178,8 @@ struct lbm_ctx {
     int npair = 0;
     int nlink = 0;
     int V = 1;                      // nodes per lane of the fused kernel
-    bool aligned_pull = true;       // see StepBufs
+    bool aligned_pull = true;
+    bool dense = false;             // FP32/16-bit: per-node pass over every node (see StepBufs)       // see StepBufs
     unsigned fast_lds = 0;
 
     // force history (device + host copy)

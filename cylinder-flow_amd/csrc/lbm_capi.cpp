@@ -422,7 +422,6 @@ static int init_plain(lbm_ctx *c) {
                 c->near_nodes += near;
             }
     const int V = c->V;
-    const Shape sh = lane_shape(g, V);
     const bool sf = !c->sfrac.empty();
     auto forced = [&](int x, int y, int z) {   // a node whose links or solid fraction carry force
         if (x >= g.nx) return false;
@@ -430,20 +429,23 @@ static int init_plain(lbm_ctx *c) {
         return flags[node] == FL_NEAR ||
                (sf && flags[node] != FL_SOLID && c->sfrac[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] > 0.0);
     };
-    // FP64 per-node kernel: a force partial slot per block holding a forced node
+    // per-node kernels over every node (FP64 k_fused, dense mode k_dense_v; one node per lane):
+    // a force partial slot per block holding a forced node
     std::vector<int> slot;
     c->nslots = 0;
-    if (c->prec == LBM_FP64) {
-        slot.assign((size_t)sh.gx * g.ny * g.nz, -1);
+    auto block_slots = [&]() {
+        const Shape s1 = lane_shape(g, 1);
+        slot.assign((size_t)s1.gx * g.ny * g.nz, -1);
+        c->nslots = 0;
         for (int z = 0; z < g.nz; z++)
             for (int y = 0; y < g.ny; y++)
-                for (int bxi = 0; bxi < sh.gx; bxi++) {
+                for (int bxi = 0; bxi < s1.gx; bxi++) {
                     bool any = false;
-                    for (int x = bxi * sh.bx * V; x < std::min(g.nx, (bxi + 1) * sh.bx * V) && !any; x++)
-                        any = forced(x, y, z);
-                    if (any) slot[(size_t)bxi + (size_t)sh.gx * ((size_t)y + (size_t)g.ny * z)] = c->nslots++;
+                    for (int x = bxi * s1.bx; x < std::min(g.nx, (bxi + 1) * s1.bx) && !any; x++) any = forced(x, y, z);
+                    if (any) slot[(size_t)bxi + (size_t)s1.gx * ((size_t)y + (size_t)g.ny * z)] = c->nslots++;
                 }
-    }
+    };
+    if (c->prec == LBM_FP64) block_slots();
     // FP32/16-bit: per fluid node (row-major padded indices), either
     //  * the link list: mask neighbours whose bounce-back is pre-applied by copying h_opp(i)
     //    into the solid source's slot i before the packed path (exact when opposite
@@ -461,11 +463,13 @@ static int init_plain(lbm_ctx *c) {
         const bool copies_exact = c->prec == LBM_FP32 || c->cfg.store_uref == 0.0;
         std::vector<uint32_t> fix, fixlk, link, linklk;
         std::vector<int> lslot;
+        size_t fluid = 0;
         for (int z = 0; z < g.nz; z++)
             for (int y = 0; y < g.ny; y++)
                 for (int x = 0; x < g.nx; x++) {
                     const size_t node = (size_t)x + (size_t)g.nxp * ((size_t)y + (size_t)g.ny * z);
                     if (flags[node] == FL_SOLID) continue;
+                    fluid++;
                     const bool f = forced(x, y, z), near = flags[node] == FL_NEAR;
                     const bool sfr = sf && c->sfrac[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] > 0.0;
                     const bool edge = (g.xper && (x == 0 || x == g.nx - 1)) || (x == g.nx - 1 && g.nx % V != 0);
@@ -496,6 +500,20 @@ static int init_plain(lbm_ctx *c) {
                         lslot.push_back(k);
                     }
                 }
+        // Dense mode: when the fix list holds more than a quarter of the fluid nodes (a solid
+        // fraction over a noisy field: 0.5 of them on the 192x96x96 adjoint workload), one
+        // per-node pass over every node costs less than the packed pass plus the fix-up of the
+        // list (measured 67 + 100 us per step there).  LBM_DENSE=1 / 0 forces the choice.
+        const char *dense_env = std::getenv("LBM_DENSE");
+        c->dense = dense_env ? dense_env[0] == '1' : (!all_slow && fix.size() * 4 > fluid);
+        if (c->dense) {
+            fix.clear();
+            fixlk.clear();
+            link.clear();
+            linklk.clear();
+            lslot.clear();
+            block_slots();
+        }
         if (fix.size() > (size_t)INT32_MAX || link.size() > (size_t)INT32_MAX)
             return set_err(c, LBM_ERR_INVALID, "too many boundary nodes");
         c->nfix = (int)fix.size();
@@ -663,6 +681,7 @@ StepBufs step_bufs(const lbm_ctx *c, const void *A, void *B) {
     b.sfrac = c->dsfrac;
     b.uin = c->duin;
     b.block_slot = c->block_slot;
+    b.dense = c->dense;
     b.partials = c->partials;
     b.macros = c->macros;
     b.aligned_pull = c->aligned_pull;

@@ -328,6 +328,7 @@ template <int Q, bool NT>
 __global__ void __launch_bounds__(256) k_ref_stream(RefGeo r, const double *__restrict__ f, double *__restrict__ ft) {
     using L = Lat<Q>;
     REF_INTERIOR_LANE(r, i, j, k);
+    (void)n;
     static_for<0, Q>([&](auto I) {
         constexpr int ii = I;
         const double h = f[RIDX(r, ii, i - L::ex(ii), j - L::ey(ii), k - L::ez(ii))];

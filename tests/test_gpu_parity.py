@@ -409,3 +409,38 @@ def test_kernel_timed_steps_equal_plain_steps(pkg, prec):
                 s.step(12)
             out.append((s.populations(), s.forces()))
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("prec,Q,periodic", [("fp32", 27, False), ("fp16s", 27, True), ("bf16s", 9, False),
+                                             ("fp32", 9, True)])
+def test_dense_mode_equals_list_path(pkg, monkeypatch, prec, Q, periodic):
+    """Dense mode (one per-node pass over every node, chosen when more than a quarter of the
+    fluid nodes need the per-node path) against the packed pass + bounce-back copies + fix-up
+    list on a mask with a noisy solid fraction: populations and fields bit for bit, forces up to
+    their summation order (per block instead of per node, 1e-12)."""
+    L = pkg.lbm
+    shape = (12, 20, 70) if Q == 27 else (1, 40, 90)
+    nz, ny, nx = shape
+    zz, yy, xx = np.mgrid[0:nz, 0:ny, 0:nx]
+    r2 = (xx - 20.0) ** 2 + (yy - ny / 2 + 0.5) ** 2 + ((zz - nz / 2 + 0.5) ** 2 if Q == 27 else 0)
+    mask = (r2 <= 9.0).astype(np.uint8)
+    sf = np.clip(np.exp(-r2 / 40.0) + 0.01 * np.random.default_rng(7).standard_normal(shape), 0.0, 1.0)
+    sf = np.where(mask == 1, 0.0, sf).astype(np.float32).astype(np.float64)
+    g = L.make_config(Q, nx, ny, nz, tau0=0.05, u0=0.05, precision=L.PRECISIONS[prec], body=L.BODY_MASK,
+                      perturb=1e-3, force_interval=3, inlet=L.INLET_PARABOLIC,
+                      periodic=(False, periodic, periodic and Q == 27))
+    out = []
+    for dense in ("0", "1"):
+        monkeypatch.setenv("LBM_DENSE", dense)
+        print(f"[parity] dense={dense} {prec} Q={Q} {shape} periodic={periodic}", flush=True)
+        with pkg.Solver(g) as s:
+            s.set_mask(mask)
+            s.set_solid_fraction(sf)
+            s.initialize()
+            s.step(13)
+            out.append((s.populations(), np.stack(s.fields()), s.forces()))
+    (pa, fa, Fa), (pb, fb, Fb) = out
+    assert np.array_equal(pa, pb)
+    assert np.array_equal(fa, fb)
+    assert Fa.shape == Fb.shape == (4, 4) and np.array_equal(Fa[:, 0], Fb[:, 0])
+    assert np.abs(Fa[:, 1:] - Fb[:, 1:]).max() <= 1e-12 * np.abs(Fa[:, 1:]).max()
