@@ -268,6 +268,8 @@ def main():
     ap.add_argument("--accuracy-window", type=int, default=None, help="first step of the statistics window")
     ap.add_argument("--default-configs", action="store_true",
                     help="instead of the benchmark, time the reference's default 3D and 2D configurations")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="instead of the benchmark, time the adjoint leg alone (forward and reverse sweep)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the whole domain runs on a single MI355X; "
@@ -281,6 +283,9 @@ def main():
         return
     if a.default_configs:
         print(json.dumps({"default_configs": default_configs(pkg)}), flush=True)
+        return
+    if a.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_ratio(pkg)}), flush=True)
         return
 
     log(f"fp16s leg: {K} steps after {Wm} warm-up")
