@@ -15,8 +15,8 @@
 //      transposes of the regularisation and of Zou-He at inlet nodes; accumulate dJ/ds(a)
 //      and dJ/du_in(row).
 // Traffic per node: Q reads of H, Q reads of mu_{m+1}, Q writes of mu_m.  H is a population
-// buffer of the forward (read through its pull); the mu arrays are planar, one plane per
-// direction: mu[i*plane + node].
+// buffer of the forward (read through its pull); the mu arrays use the populations' layout,
+// mu[pel(i, x, row)] (row-interleaved by default).
 #include "fused_common.hpp"
 
 namespace lbm {
@@ -31,10 +31,11 @@ __global__ void __launch_bounds__(256)
     const Geo &g = a.f.g;
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
     if (x >= g.nx) return;
-    const uint32_t node = (uint32_t)x + (uint32_t)g.nxp * ((uint32_t)y + (uint32_t)g.ny * (uint32_t)z);
+    const uint32_t row = (uint32_t)y + (uint32_t)g.ny * (uint32_t)z;
+    const uint32_t node = (uint32_t)x + (uint32_t)g.nxp * row;
     const uint8_t fl = flags[node];
     if (fl == FL_SOLID) {
-        static_for<0, Q>([&](auto I) { mu[(size_t)I * g.plane + node] = (C)0; });
+        static_for<0, Q>([&](auto I) { mu[pel(g, I, (uint32_t)x, row)] = (C)0; });
         return;
     }
     const C an = (C)a.alpha_next;
@@ -69,11 +70,12 @@ __global__ void __launch_bounds__(256)
                 }
             }
             if (wall) {
-                l = mun[(size_t)o * g.plane + node];
+                l = mun[pel(g, o, (uint32_t)x, row)];
             } else if (!out) {
-                const uint32_t d = (uint32_t)dx + (uint32_t)g.nxp * ((uint32_t)dy + (uint32_t)g.ny * (uint32_t)dz);
-                if (fl == FL_NEAR && flags[d] == FL_SOLID) l = mun[(size_t)o * g.plane + node] + an * (C)(2 * ex);
-                else l = mun[(size_t)i * g.plane + d];
+                const uint32_t drow = (uint32_t)dy + (uint32_t)g.ny * (uint32_t)dz;
+                const uint32_t d = (uint32_t)dx + (uint32_t)g.nxp * drow;
+                if (fl == FL_NEAR && flags[d] == FL_SOLID) l = mun[pel(g, o, (uint32_t)x, row)] + an * (C)(2 * ex);
+                else l = mun[pel(g, i, (uint32_t)dx, drow)];
             }
         }
         // outlet: (nx-1, y+e_y, z+e_z) pulls h_i from this node when e_ix < 0
@@ -93,7 +95,7 @@ __global__ void __launch_bounds__(256)
                         else ok = false;
                     }
                 }
-                if (ok) l += mun[(size_t)i * g.plane + (uint32_t)x + (uint32_t)g.nxp * ((uint32_t)dy + (uint32_t)g.ny * (uint32_t)dz)];
+                if (ok) l += mun[pel(g, i, (uint32_t)x, (uint32_t)dy + (uint32_t)g.ny * (uint32_t)dz)];
             }
         }
         lam[i] = l;
@@ -204,7 +206,7 @@ __global__ void __launch_bounds__(256)
         });
         dJdu[y + (size_t)g.ny * z] += du;
     }
-    static_for<0, Q>([&](auto I) { mu[(size_t)I * g.plane + node] = m[I]; });
+    static_for<0, Q>([&](auto I) { mu[pel(g, I, (uint32_t)x, row)] = m[I]; });
 }
 
 // initial condition H_0 = feq(0, (u_in(y,z) + perturb*xi, 0, 0)): dJ/du_in(row) +=
@@ -219,8 +221,9 @@ __global__ void __launch_bounds__(256)
     using L = Lat<Q>;
     const int y = blockIdx.y, z = blockIdx.z;
     double acc = 0.0;
+    const uint32_t row = (uint32_t)y + (uint32_t)g.ny * (uint32_t)z;
     for (int x = threadIdx.x; x < g.nx; x += blockDim.x) {
-        const uint32_t node = (uint32_t)x + (uint32_t)g.nxp * ((uint32_t)y + (uint32_t)g.ny * (uint32_t)z);
+        const uint32_t node = (uint32_t)x + (uint32_t)g.nxp * row;
         const uint8_t fl = flags[node];
         if (fl == FL_SOLID) continue;
         const size_t phys = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z);
@@ -253,11 +256,12 @@ __global__ void __launch_bounds__(256)
                     }
                 }
                 if (wall) {
-                    l = (double)mun[(size_t)o * g.plane + node];
+                    l = (double)mun[pel(g, o, (uint32_t)x, row)];
                 } else if (!out) {
-                    const uint32_t d = (uint32_t)dx + (uint32_t)g.nxp * ((uint32_t)dy + (uint32_t)g.ny * (uint32_t)dz);
-                    if (fl == FL_NEAR && flags[d] == FL_SOLID) l = (double)mun[(size_t)o * g.plane + node] + alpha1 * 2.0 * ex;
-                    else l = (double)mun[(size_t)i * g.plane + d];
+                    const uint32_t drow = (uint32_t)dy + (uint32_t)g.ny * (uint32_t)dz;
+                    const uint32_t d = (uint32_t)dx + (uint32_t)g.nxp * drow;
+                    if (fl == FL_NEAR && flags[d] == FL_SOLID) l = (double)mun[pel(g, o, (uint32_t)x, row)] + alpha1 * 2.0 * ex;
+                    else l = (double)mun[pel(g, i, (uint32_t)dx, drow)];
                 }
             }
             if constexpr (ex < 0) {
@@ -276,7 +280,7 @@ __global__ void __launch_bounds__(256)
                             else ok = false;
                         }
                     }
-                    if (ok) l += (double)mun[(size_t)i * g.plane + (uint32_t)x + (uint32_t)g.nxp * ((uint32_t)dy + (uint32_t)g.ny * (uint32_t)dz)];
+                    if (ok) l += (double)mun[pel(g, i, (uint32_t)x, (uint32_t)dy + (uint32_t)g.ny * (uint32_t)dz)];
                 }
             }
             acc += l * L::w(i) * (3.0 * ex + 9.0 * ex * u * ex - 3.0 * u);
