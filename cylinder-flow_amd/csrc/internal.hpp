@@ -111,7 +111,8 @@ struct RefGeo {
 hipError_t ref_init(const RefGeo &r, double *f, double *ft, double *mac, uint8_t *usr, const int32_t *ran,
                     int body, double cx, double cy, double cz, double r2, uint64_t seed, hipStream_t s);
 hipError_t ref_fills(const RefGeo &r, double *f, double *ft, hipStream_t s);
-hipError_t ref_usrbc(const RefGeo &r, double *f, const uint8_t *usr, hipStream_t s);
+// list (may be null: sweep the interior): the nodes usrbc can change, nlist of them
+hipError_t ref_usrbc(const RefGeo &r, double *f, const uint8_t *usr, const uint32_t *list, int nlist, hipStream_t s);
 hipError_t ref_mem(const RefGeo &r, const double *f, const uint8_t *usr, double *partials, int nblk,
                    double *hist, int rec, long step, hipStream_t s);
 // fpre (may be null): receives the pulled populations, i.e. the reference's f after Update
@@ -190,6 +191,8 @@ struct lbm_ctx {
     lbm::RefGeo rg{};
     double *rf = nullptr, *rft = nullptr, *rmac = nullptr, *rprof = nullptr, *rfpre = nullptr;
     uint8_t *rusr = nullptr;
+    uint32_t *rsurf = nullptr;      // interior solid nodes with a fluid neighbour (usrbc's work list)
+    int nsurf = 0;
 
     bool initialized = false;
     long step = 0;

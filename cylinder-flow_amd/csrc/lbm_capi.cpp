@@ -92,7 +92,7 @@ extern "C" const char *lbm_last_error(const lbm_ctx *ctx) { return ctx ? ctx->er
 
 static void free_all(lbm_ctx *c) {
     void *ptrs[] = {c->A, c->B, c->flags, c->dsfrac, c->duin, c->macros, c->block_slot, c->partials, c->hist, c->fix,
-                    c->fixlk, c->link, c->linklk, c->link_slot, c->lpair, c->rf, c->rft, c->rmac, c->rprof, c->rusr, c->rfpre};
+                    c->fixlk, c->link, c->linklk, c->link_slot, c->lpair, c->rf, c->rft, c->rmac, c->rprof, c->rusr, c->rfpre, c->rsurf};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c->A = c->B = nullptr;
@@ -106,6 +106,8 @@ static void free_all(lbm_ctx *c) {
     c->npair = 0;
     c->nfix = c->nlink = 0;
     c->rf = c->rft = c->rmac = c->rprof = c->rfpre = nullptr;
+    c->rsurf = nullptr;
+    c->nsurf = 0;
     c->rusr = nullptr;
 }
 
@@ -631,18 +633,30 @@ static int init_ref(lbm_ctx *c) {
     lbm_lattice_table(Q, e.data(), nullptr, nullptr);
     const int g = r.g, klo = c->D == 3 ? g : 0, khi = c->D == 3 ? r.lz - g : 1;
     c->solid_nodes = 0;
+    std::vector<uint32_t> surf;   // usrbc's work list: interior solid nodes with a fluid neighbour
     for (int k = klo; k < khi; k++)
         for (int j = g; j < r.ly - g; j++)
             for (int i = g; i < r.lx - g; i++) {
                 const long n = i + (long)r.px * (j + (long)r.ly * k);
                 if (usr[n] != 0) continue;
                 c->solid_nodes++;
+                bool wet = false;
                 for (int q = 1; q < Q; q++) {
                     const long d = e[3 * q] + (long)r.px * (e[3 * q + 1] + (long)r.ly * e[3 * q + 2]);
                     if (usr[n + d] == 1 && usr[n + 2 * d] == 0)
                         return set_err(c, LBM_ERR_INVALID, "reference mode: body has a one-node fluid gap along a lattice link (usrbc would race)");
+                    wet = wet || usr[n + d] == 1;
                 }
+                if (wet) surf.push_back((uint32_t)n);
             }
+    // 3D only: the 2D box (65 536 nodes by default) is launch-bound, and its step measured 7 %
+    // slower with the list than with the interior sweep
+    if (c->D == 3 && r.plane <= UINT32_MAX) {
+        c->nsurf = (int)surf.size();
+        HIP_TRY(c, hipMalloc(&c->rsurf, sizeof(uint32_t) * std::max<size_t>(surf.size(), 1)));
+        if (!surf.empty())
+            HIP_TRY(c, hipMemcpy(c->rsurf, surf.data(), sizeof(uint32_t) * surf.size(), hipMemcpyHostToDevice));
+    }
     const int nblk = ref_mem_blocks(r);
     HIP_TRY(c, hipMalloc(&c->partials, sizeof(double) * 3 * nblk));
     c->nslots = nblk;
@@ -703,7 +717,7 @@ static int step_n(lbm_ctx *c, int n, hipEvent_t *evs = nullptr) {
             c->step++;
             const bool force = fi > 0 && c->step % fi == 0;
             HIP_TRY(c, ref_fills(c->rg, c->rf, c->rft, c->stream));
-            HIP_TRY(c, ref_usrbc(c->rg, c->rf, c->rusr, c->stream));
+            HIP_TRY(c, ref_usrbc(c->rg, c->rf, c->rusr, c->rsurf, c->nsurf, c->stream));
             if (force) HIP_TRY(c, ref_mem(c->rg, c->rf, c->rusr, c->partials, c->nslots, c->hist, c->nrec++, c->step, c->stream));
             HIP_TRY(c, ref_fused(c->rg, c->rf, c->rft, k == n - 1 ? c->rmac : nullptr, c->rusr, c->rprof,
                                  k == n - 1 ? c->rfpre : nullptr, c->stream));
@@ -768,7 +782,7 @@ extern "C" int lbm_boundary(lbm_ctx *c) {
     if (rc != LBM_OK) return rc;
     const int fi = c->cfg.force_interval;
     HIP_TRY(c, ref_fills(c->rg, c->rf, c->rft, c->stream));
-    HIP_TRY(c, ref_usrbc(c->rg, c->rf, c->rusr, c->stream));
+    HIP_TRY(c, ref_usrbc(c->rg, c->rf, c->rusr, c->rsurf, c->nsurf, c->stream));
     if (fi > 0 && c->step % fi == 0)
         HIP_TRY(c, ref_mem(c->rg, c->rf, c->rusr, c->partials, c->nslots, c->hist, c->nrec++, c->step, c->stream));
     c->stage = 2;

@@ -108,8 +108,25 @@ __device__ __forceinline__ void copy2(double *__restrict__ f, double *__restrict
 }
 
 
+// the 9 directions a 3D face fill writes (those pointing into the domain from that face)
+template <int FACE>
+constexpr bool on_face(int ii) {
+    using L = Lat<27>;
+    return FACE == 0 ? L::ey(ii) > 0 : FACE == 1 ? L::ey(ii) < 0 : FACE == 2 ? L::ez(ii) > 0
+         : FACE == 3 ? L::ez(ii) < 0 : FACE == 4 ? L::ex(ii) > 0 : L::ex(ii) < 0;
+}
+template <int FACE>
+constexpr int face_rank(int ii) {
+    int k = 0;
+    for (int jj = 0; jj < ii; jj++) k += on_face<FACE>(jj);
+    return k;
+}
+
 // 3D fills, one kernel per face: 3D MPI+KOKKOS+MPIIO/lbm.cpp:278-347.
-// face: 0 front (y lo), 1 back (y hi), 2 bottom (z lo), 3 top (z hi), 4 inlet, 5 outlet
+// face: 0 front (y lo), 1 back (y hi), 2 bottom (z lo), 3 top (z hi), 4 inlet, 5 outlet.
+// One lane per (face position, direction): blockIdx.z picks the direction among the face's
+// nine.  Within one fill no copy reads a slot that another copy of the same fill writes (the
+// sources lie inside the ghost layer's inner neighbour planes), so the directions need no order.
 template <int FACE>
 __global__ void __launch_bounds__(256) k_ref_fill3(RefGeo r, double *__restrict__ f, double *__restrict__ ft) {
     using L = Lat<27>;
@@ -120,9 +137,12 @@ __global__ void __launch_bounds__(256) k_ref_fill3(RefGeo r, double *__restrict_
     if (FACE <= 1) { if (a >= r.lx - g + 1 || b >= r.lz - g + 1) return; }   // (i, k)
     else if (FACE <= 3) { if (a >= r.lx - g + 1 || b >= r.ly - g + 1) return; }  // (i, j)
     else { if (a >= r.ly - g + 1 || b >= r.lz - g + 1) return; }   // (j, k)
+    const int slot = (int)blockIdx.z;   // uniform over the block
     static_for<0, 27>([&](auto I) {
         constexpr int ii = I;
         constexpr int ex = L::ex(ii), ey = L::ey(ii), ez = L::ez(ii), bb = L::opp(ii);
+        if constexpr (!on_face<FACE>(ii)) return;
+        if (face_rank<FACE>(ii) != slot) return;
         if constexpr (FACE == 0 && ey > 0)
             copy2(f, ft, RIDX(r, ii, a, g - 1, b), RIDX(r, bb, a + 2 * ex, g + 1, b + 2 * ez));
         if constexpr (FACE == 1 && ey < 0)
@@ -189,6 +209,26 @@ __global__ void __launch_bounds__(256) k_ref_usrbc(RefGeo r, double *__restrict_
         if (usr[nb] == 1) {
             f[RIDX(r, ii, i, j, k)] = f[RIDX(r, L::opp(ii), i + 2 * L::ex(ii), j + 2 * L::ey(ii), k + 2 * L::ez(ii))];
         }
+    });
+}
+
+// usrbc over a list of the nodes it can change (interior solid nodes with a fluid neighbour,
+// node indices built once from usr): the same per-node work as k_ref_usrbc without sweeping
+// the interior
+template <int Q>
+__global__ void __launch_bounds__(256) k_ref_usrbc_list(RefGeo r, double *__restrict__ f, const uint8_t *__restrict__ usr,
+                                                        const uint32_t *__restrict__ list, int nlist) {
+    using L = Lat<Q>;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nlist) return;
+    const uint32_t node = list[t], row = node / (uint32_t)r.px;
+    const int i = (int)(node - row * (uint32_t)r.px), j = (int)(row % (uint32_t)r.ly), k = (int)(row / (uint32_t)r.ly);
+    const long n = (long)node;
+    static_for<0, Q>([&](auto I) {
+        constexpr int ii = I;
+        const long nb = n + L::ex(ii) + (long)r.px * (L::ey(ii) + (long)r.ly * L::ez(ii));
+        if (usr[nb] == 1)
+            f[RIDX(r, ii, i, j, k)] = f[RIDX(r, L::opp(ii), i + 2 * L::ex(ii), j + 2 * L::ey(ii), k + 2 * L::ez(ii))];
     });
 }
 
@@ -412,12 +452,12 @@ hipError_t ref_init(const RefGeo &r, double *f, double *ft, double *mac, uint8_t
 hipError_t ref_fills(const RefGeo &r, double *f, double *ft, hipStream_t s) {
     if (r.D == 3) {
         const int nI = r.lx - 2 * (r.g - 1), nJ = r.ly - 2 * (r.g - 1), nK = r.lz - 2 * (r.g - 1);
-        hipLaunchKernelGGL(k_ref_fill3<0>, dim3((nI + 255) / 256, nK), dim3(256), 0, s, r, f, ft);
-        hipLaunchKernelGGL(k_ref_fill3<1>, dim3((nI + 255) / 256, nK), dim3(256), 0, s, r, f, ft);
-        hipLaunchKernelGGL(k_ref_fill3<2>, dim3((nI + 255) / 256, nJ), dim3(256), 0, s, r, f, ft);
-        hipLaunchKernelGGL(k_ref_fill3<3>, dim3((nI + 255) / 256, nJ), dim3(256), 0, s, r, f, ft);
-        hipLaunchKernelGGL(k_ref_fill3<4>, dim3((nJ + 255) / 256, nK), dim3(256), 0, s, r, f, ft);
-        hipLaunchKernelGGL(k_ref_fill3<5>, dim3((nJ + 255) / 256, nK), dim3(256), 0, s, r, f, ft);
+        hipLaunchKernelGGL(k_ref_fill3<0>, dim3((nI + 255) / 256, nK, 9), dim3(256), 0, s, r, f, ft);
+        hipLaunchKernelGGL(k_ref_fill3<1>, dim3((nI + 255) / 256, nK, 9), dim3(256), 0, s, r, f, ft);
+        hipLaunchKernelGGL(k_ref_fill3<2>, dim3((nI + 255) / 256, nJ, 9), dim3(256), 0, s, r, f, ft);
+        hipLaunchKernelGGL(k_ref_fill3<3>, dim3((nI + 255) / 256, nJ, 9), dim3(256), 0, s, r, f, ft);
+        hipLaunchKernelGGL(k_ref_fill3<4>, dim3((nJ + 255) / 256, nK, 9), dim3(256), 0, s, r, f, ft);
+        hipLaunchKernelGGL(k_ref_fill3<5>, dim3((nJ + 255) / 256, nK, 9), dim3(256), 0, s, r, f, ft);
     } else {
         hipLaunchKernelGGL(k_ref_fill2<0>, dim3((r.ly + 255) / 256), dim3(256), 0, s, r, f, ft);
         hipLaunchKernelGGL(k_ref_fill2<1>, dim3((r.ly + 255) / 256), dim3(256), 0, s, r, f, ft);
@@ -427,7 +467,14 @@ hipError_t ref_fills(const RefGeo &r, double *f, double *ft, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t ref_usrbc(const RefGeo &r, double *f, const uint8_t *usr, hipStream_t s) {
+hipError_t ref_usrbc(const RefGeo &r, double *f, const uint8_t *usr, const uint32_t *list, int nlist, hipStream_t s) {
+    if (list) {   // the listed nodes only (none: nothing to do)
+        if (nlist == 0) return hipSuccess;
+        const dim3 grid((unsigned)((nlist + 255) / 256));
+        if (r.Q == 9) hipLaunchKernelGGL(k_ref_usrbc_list<9>, grid, dim3(256), 0, s, r, f, usr, list, nlist);
+        else hipLaunchKernelGGL(k_ref_usrbc_list<27>, grid, dim3(256), 0, s, r, f, usr, list, nlist);
+        return hipGetLastError();
+    }
     if (r.Q == 9) hipLaunchKernelGGL(k_ref_usrbc<9>, interior_grid(r), dim3(256), 0, s, r, f, usr);
     else hipLaunchKernelGGL(k_ref_usrbc<27>, interior_grid(r), dim3(256), 0, s, r, f, usr);
     return hipGetLastError();
