@@ -78,7 +78,7 @@ struct StepBufs {
     bool aligned_pull;          // 16-bit, two nodes per lane: aligned loads + lane shifts (k_fast_v)
     unsigned fast_lds;          // dynamic LDS bytes per k_fast_v block (caps its blocks per CU; 0: none)
     hipEvent_t ev_bulk[2];      // when set: recorded just before and after the step's bulk kernel
-                                // (k_fast_v, or k_fused in FP64), on the step's stream
+                                // (k_fast_v, k_dense_v, or k_fused in FP64), on the step's stream
 };
 hipError_t plain_step(int Q, int prec, int V, const StepBufs &b, bool force, bool macros_on, const FusedArgs &a,
                       hipStream_t s);

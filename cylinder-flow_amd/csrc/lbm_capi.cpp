@@ -502,12 +502,20 @@ static int init_plain(lbm_ctx *c) {
                         lslot.push_back(k);
                     }
                 }
-        // Dense mode: when the fix list holds more than a quarter of the fluid nodes (a solid
-        // fraction over a noisy field: 0.5 of them on the 192x96x96 adjoint workload), one
-        // per-node pass over every node costs less than the packed pass plus the fix-up of the
-        // list (measured 67 + 100 us per step there).  LBM_DENSE=1 / 0 forces the choice.
+        // Dense mode: one per-node pass over every node instead of the copies, the packed pass
+        // and the fix-up of the list, chosen
+        //  * when the fix list holds more than a quarter of the fluid nodes (a solid fraction
+        //    over a noisy field: half of them on the 192x96x96 adjoint workload, where the
+        //    list path took 67 + 100 us per step and the dense pass 67 us), or
+        //  * when a population buffer is at most 16 MB: such steps are launch-bound, and one
+        //    launch fewer per step counts (D2Q9 512x128 with the drag every step: 6.7 / 6.4 us
+        //    per step in FP32 / FP16 against 8.7 / 8.8; on D3Q27 512x128x128 the dense pass
+        //    is as fast in FP32 but 1.5 times slower in FP16, so large domains keep the list).
+        // LBM_DENSE=1 / 0 forces the choice.
+        const size_t pop_bytes = (size_t)(c->prec == LBM_FP32 ? 4 : 2) * Q * g.plane;
+        const bool small = pop_bytes <= ((size_t)16 << 20);
         const char *dense_env = std::getenv("LBM_DENSE");
-        c->dense = dense_env ? dense_env[0] == '1' : (!all_slow && fix.size() * 4 > fluid);
+        c->dense = dense_env ? dense_env[0] == '1' : (!all_slow && (fix.size() * 4 > fluid || small));
         if (c->dense) {
             fix.clear();
             fixlk.clear();

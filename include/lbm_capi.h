@@ -124,8 +124,8 @@ int lbm_step(lbm_ctx *ctx, int n);
 /* the same, bracketed by HIP events on the context stream; *ms = elapsed milliseconds */
 int lbm_step_timed(lbm_ctx *ctx, int n, double *ms);
 /* n steps as lbm_step, timed with HIP events on the context stream: *ms_steps = the n steps,
- * *ms_bulk = the sum over the steps of the bulk kernel alone (k_fast_v, or k_fused for FP64
- * storage), between events recorded just before and after its launch.  Plain mode only,
+ * *ms_bulk = the sum over the steps of the bulk kernel alone (k_fast_v; k_dense_v in dense mode;
+ * k_fused for FP64 storage), between events recorded just before and after its launch.  Plain mode only,
  * 1 <= n <= 100000.  A measurement entry point with no counterpart in the reference. */
 int lbm_step_kernel_timed(lbm_ctx *ctx, int n, double *ms_steps, double *ms_bulk);
 /* One time step in the reference's stages, callable one at a time and only in this order

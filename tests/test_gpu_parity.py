@@ -338,6 +338,7 @@ def _fast_vs_per_node(pkg, monkeypatch, prec, V, Q, shape, kw, store_uref):
     g = L.make_config(Q, nx, ny, nz, tau0=0.05, u0=0.05, precision=L.PRECISIONS[prec], bc_mode=L.BC_PLAIN,
                       perturb=1e-3, force_interval=3, store_uref=store_uref, **kw)
     monkeypatch.setenv("LBM_NODES_PER_LANE", str(V))
+    monkeypatch.setenv("LBM_DENSE", "0")     # the packed path, not the dense pass of small domains
     out = []
     for slow in ("0", "1"):
         monkeypatch.setenv("LBM_FORCE_SLOW_PATH", slow)
@@ -375,6 +376,7 @@ def test_launch_variants_give_identical_results(pkg, monkeypatch, prec, env, val
     g = L.make_config(27, 512, 12, 10, tau0=0.05, u0=0.05, precision=L.PRECISIONS[prec], body=L.BODY_ZCYL,
                       center=(100.0, 5.5, 0.0), r2=9.0, perturb=1e-3, force_interval=4,
                       periodic=(False, periodic, periodic))
+    monkeypatch.setenv("LBM_DENSE", "0")     # the packed path, not the dense pass of small domains
     out = []
     for v in values:
         if v is None:
