@@ -128,7 +128,7 @@ constexpr int face_rank(int ii) {
 // nine.  Within one fill no copy reads a slot that another copy of the same fill writes (the
 // sources lie inside the ghost layer's inner neighbour planes), so the directions need no order.
 template <int FACE>
-__global__ void __launch_bounds__(256) k_ref_fill3(RefGeo r, double *__restrict__ f, double *__restrict__ ft) {
+__device__ __forceinline__ void fill3(const RefGeo &r, double *__restrict__ f, double *__restrict__ ft, int slot) {
     using L = Lat<27>;
     const int a = blockIdx.x * blockDim.x + threadIdx.x + r.g - 1;   // first in-face coordinate
     const int b = blockIdx.y + r.g - 1;                              // second in-face coordinate
@@ -137,7 +137,6 @@ __global__ void __launch_bounds__(256) k_ref_fill3(RefGeo r, double *__restrict_
     if (FACE <= 1) { if (a >= r.lx - g + 1 || b >= r.lz - g + 1) return; }   // (i, k)
     else if (FACE <= 3) { if (a >= r.lx - g + 1 || b >= r.ly - g + 1) return; }  // (i, j)
     else { if (a >= r.ly - g + 1 || b >= r.lz - g + 1) return; }   // (j, k)
-    const int slot = (int)blockIdx.z;   // uniform over the block
     static_for<0, 27>([&](auto I) {
         constexpr int ii = I;
         constexpr int ex = L::ex(ii), ey = L::ey(ii), ez = L::ez(ii), bb = L::opp(ii);
@@ -157,11 +156,22 @@ __global__ void __launch_bounds__(256) k_ref_fill3(RefGeo r, double *__restrict_
     });
 }
 
+// The two faces of one axis in one launch (blockIdx.z < 9: face FA, else FB; the slot picks the
+// direction): their copies touch disjoint slots (FA writes and reads only planes near its own
+// face, FB near the opposite one, at least 3 nodes apart), so they need no order between them;
+// the reference's order between the axes stays (y faces, then z, then x).
+template <int FA, int FB>
+__global__ void __launch_bounds__(256) k_ref_fill3(RefGeo r, double *__restrict__ f, double *__restrict__ ft) {
+    const int z = (int)blockIdx.z;   // uniform over the block
+    if (z < 9) fill3<FA>(r, f, ft, z);
+    else fill3<FB>(r, f, ft, z - 9);
+}
+
 // 2D fills: MPI Kokkos/lbm.cpp:315-345 (stream3 inlet, stream3 outlet, stream4 bottom,
 // stream4 top).  Columns i-2 = -1 and i+2 = lx wrap through the flat index, as in the
 // reference's LayoutLeft views.
 template <int FACE>
-__global__ void __launch_bounds__(256) k_ref_fill2(RefGeo r, double *__restrict__ f, double *__restrict__ ft) {
+__device__ __forceinline__ void fill2(const RefGeo &r, double *__restrict__ f, double *__restrict__ ft) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int lx = r.lx, ly = r.ly;
     if (FACE <= 1) {
@@ -190,6 +200,16 @@ __global__ void __launch_bounds__(256) k_ref_fill2(RefGeo r, double *__restrict_
             copy2(f, ft, RIDX(r, 8, i, ly - 2, 0), RIDX(r, 6, i + 2, ly - 4, 0));
         }
     }
+}
+
+// Inlet and outlet (FA, FB = 0, 1), or bottom and top (2, 3), in one launch (blockIdx.y picks
+// the face): the pair writes and reads columns 1 and 2 against lx-2 and lx-3, or rows 1 and
+// 2..4 against ly-2 and ly-5..ly-3 (ly >= 7), so its two faces touch disjoint slots and need
+// no order between them; inlet/outlet still run before bottom/top, as in the reference.
+template <int FA, int FB>
+__global__ void __launch_bounds__(256) k_ref_fill2(RefGeo r, double *__restrict__ f, double *__restrict__ ft) {
+    if (blockIdx.y == 0) fill2<FA>(r, f, ft);
+    else fill2<FB>(r, f, ft);
 }
 
 // usrbc: 3D MPI+KOKKOS+MPIIO/lbm.cpp:350-356 ; MPI Kokkos/lbm.cpp:347-353 (in place; the
@@ -452,17 +472,12 @@ hipError_t ref_init(const RefGeo &r, double *f, double *ft, double *mac, uint8_t
 hipError_t ref_fills(const RefGeo &r, double *f, double *ft, hipStream_t s) {
     if (r.D == 3) {
         const int nI = r.lx - 2 * (r.g - 1), nJ = r.ly - 2 * (r.g - 1), nK = r.lz - 2 * (r.g - 1);
-        hipLaunchKernelGGL(k_ref_fill3<0>, dim3((nI + 255) / 256, nK, 9), dim3(256), 0, s, r, f, ft);
-        hipLaunchKernelGGL(k_ref_fill3<1>, dim3((nI + 255) / 256, nK, 9), dim3(256), 0, s, r, f, ft);
-        hipLaunchKernelGGL(k_ref_fill3<2>, dim3((nI + 255) / 256, nJ, 9), dim3(256), 0, s, r, f, ft);
-        hipLaunchKernelGGL(k_ref_fill3<3>, dim3((nI + 255) / 256, nJ, 9), dim3(256), 0, s, r, f, ft);
-        hipLaunchKernelGGL(k_ref_fill3<4>, dim3((nJ + 255) / 256, nK, 9), dim3(256), 0, s, r, f, ft);
-        hipLaunchKernelGGL(k_ref_fill3<5>, dim3((nJ + 255) / 256, nK, 9), dim3(256), 0, s, r, f, ft);
+        hipLaunchKernelGGL((k_ref_fill3<0, 1>), dim3((nI + 255) / 256, nK, 18), dim3(256), 0, s, r, f, ft);
+        hipLaunchKernelGGL((k_ref_fill3<2, 3>), dim3((nI + 255) / 256, nJ, 18), dim3(256), 0, s, r, f, ft);
+        hipLaunchKernelGGL((k_ref_fill3<4, 5>), dim3((nJ + 255) / 256, nK, 18), dim3(256), 0, s, r, f, ft);
     } else {
-        hipLaunchKernelGGL(k_ref_fill2<0>, dim3((r.ly + 255) / 256), dim3(256), 0, s, r, f, ft);
-        hipLaunchKernelGGL(k_ref_fill2<1>, dim3((r.ly + 255) / 256), dim3(256), 0, s, r, f, ft);
-        hipLaunchKernelGGL(k_ref_fill2<2>, dim3((r.lx + 255) / 256), dim3(256), 0, s, r, f, ft);
-        hipLaunchKernelGGL(k_ref_fill2<3>, dim3((r.lx + 255) / 256), dim3(256), 0, s, r, f, ft);
+        hipLaunchKernelGGL((k_ref_fill2<0, 1>), dim3((r.ly + 255) / 256, 2), dim3(256), 0, s, r, f, ft);
+        hipLaunchKernelGGL((k_ref_fill2<2, 3>), dim3((r.lx + 255) / 256, 2), dim3(256), 0, s, r, f, ft);
     }
     return hipGetLastError();
 }
