@@ -107,6 +107,7 @@ struct RefGeo {
     size_t ds, rs;               // population element (ii, i, j, k) = ii*ds + i + rs*(j + ly*k)
     double tau0, u0;
     int variant;                 // D2Q9: 0 the MPI variant, 1 the single-node variant
+    unsigned lds;                // dynamic LDS per block of the fused kernel (caps its blocks per CU; 0: none)
 };
 hipError_t ref_init(const RefGeo &r, double *f, double *ft, double *mac, uint8_t *usr, const int32_t *ran,
                     int body, double cx, double cy, double cz, double r2, uint64_t seed, hipStream_t s);

@@ -182,9 +182,13 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_ctx **out) {
     // at 3), FP16/BF16 at 6 (73.5 % against 71.7 % uncapped and 67.4 % at 4).  The cap is a
     // dynamic LDS allocation the kernel does not use.  LBM_FAST_BLOCKS_PER_CU overrides it
     // (0: no cap).  Rows of fewer than 256 lanes (narrow domains) keep the same waves per CU.
+    // Reference mode's fused kernel (FP64, D3Q27): fastest at 3 (512x128x128 sphere: 0.730 ms per
+    // step against 0.767 uncapped, 0.761 at 2, 0.751 at 4); the plain FP64 kernel gains nothing
+    // (0.628-0.630 ms at none, 3, 4; 0.701 at 2), so it stays uncapped.
     int bpc = 0;
     if (c->prec == LBM_FP32) bpc = Q == 27 ? 3 : 0;
     else if (c->prec != LBM_FP64) bpc = Q == 27 ? 4 : 6;
+    else if (ref_mode(cfg->bc_mode) && Q == 27) bpc = 3;
     if (const char *env = std::getenv("LBM_FAST_BLOCKS_PER_CU")) bpc = std::max(0, std::atoi(env));
     hipError_t e = hipSuccess;
     if (bpc > 0) {
@@ -283,6 +287,7 @@ static void ref_extents(const lbm_ctx *c, RefGeo &r) {
     r.tau0 = c->cfg.tau0;
     r.u0 = c->cfg.u0;
     r.variant = c->cfg.bc_mode == LBM_BC_REFERENCE_SN ? 1 : 0;
+    r.lds = c->fast_lds;
 }
 
 extern "C" int lbm_set_ran(lbm_ctx *c, const int32_t *ran, size_t count) {

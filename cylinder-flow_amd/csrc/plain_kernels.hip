@@ -308,11 +308,11 @@ static hipError_t step_f64(const StepBufs &b, const FusedArgs &a, hipStream_t s)
         if (e != hipSuccess) return e;
     }
     if (b.sfrac)
-        hipLaunchKernelGGL((k_fused<Q, S, FORCE, MACROS, true>), grid_of(a.g, sh), dim3(sh.bx), 0, s,
+        hipLaunchKernelGGL((k_fused<Q, S, FORCE, MACROS, true>), grid_of(a.g, sh), dim3(sh.bx), b.fast_lds, s,
                            (const double *)b.A, (double *)b.B, b.flags, (const double *)b.sfrac, b.uin, b.block_slot,
                            b.partials, b.macros, a);
     else
-        hipLaunchKernelGGL((k_fused<Q, S, FORCE, MACROS, false>), grid_of(a.g, sh), dim3(sh.bx), 0, s,
+        hipLaunchKernelGGL((k_fused<Q, S, FORCE, MACROS, false>), grid_of(a.g, sh), dim3(sh.bx), b.fast_lds, s,
                            (const double *)b.A, (double *)b.B, b.flags, (const double *)nullptr, b.uin, b.block_slot,
                            b.partials, b.macros, a);
     hipError_t e = hipGetLastError();

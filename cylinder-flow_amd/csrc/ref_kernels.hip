@@ -521,9 +521,12 @@ static hipError_t launch_rows(const RefGeo &r, hipStream_t s, Args... args) {
     return hipGetLastError();
 }
 
+// dynamic LDS of the fused kernel's blocks: the resident-block cap (RefGeo::lds, unused memory)
+template <class... A> static unsigned ref_lds(const RefGeo &r, A...) { return r.lds; }
+
 template <int Q, bool NT> struct FusedK {
     template <class... A> static void launch(dim3 g, hipStream_t s, A... a) {
-        hipLaunchKernelGGL((k_ref_fused<Q, NT>), g, dim3(256), 0, s, a...);
+        hipLaunchKernelGGL((k_ref_fused<Q, NT>), g, dim3(256), ref_lds(a...), s, a...);
     }
 };
 template <int Q, bool NT> struct StreamK {
