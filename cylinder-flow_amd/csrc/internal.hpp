@@ -114,8 +114,9 @@ hipError_t ref_init(const RefGeo &r, double *f, double *ft, double *mac, uint8_t
 hipError_t ref_fills(const RefGeo &r, double *f, double *ft, hipStream_t s);
 // list (may be null: sweep the interior): the nodes usrbc can change, nlist of them
 hipError_t ref_usrbc(const RefGeo &r, double *f, const uint8_t *usr, const uint32_t *list, int nlist, hipStream_t s);
-hipError_t ref_mem(const RefGeo &r, const double *f, const uint8_t *usr, double *partials, int nblk,
-                   double *hist, int rec, long step, hipStream_t s);
+// with_usrbc: the same pass also applies usrbc (in place of a separate ref_usrbc before it)
+hipError_t ref_mem(const RefGeo &r, double *f, const uint8_t *usr, double *partials, int nblk,
+                   double *hist, int rec, long step, bool with_usrbc, hipStream_t s);
 // fpre (may be null): receives the pulled populations, i.e. the reference's f after Update
 hipError_t ref_fused(const RefGeo &r, const double *f, double *ft, double *mac, const uint8_t *usr,
                      const double *prof, double *fpre, hipStream_t s);

@@ -717,6 +717,18 @@ StepBufs step_bufs(const lbm_ctx *c, const void *A, void *B) {
     return b;
 }
 
+// Reference mode, after the fills: usrbc and, on force steps, the momentum exchange.  In 2D
+// (launch-bound, the drag every step by default) one pass does both; in 3D usrbc runs over its
+// list and the exchange separately.
+static hipError_t ref_boundary_pass(lbm_ctx *c, bool force) {
+    if (force && c->D == 2)
+        return ref_mem(c->rg, c->rf, c->rusr, c->partials, c->nslots, c->hist, c->nrec++, c->step, true, c->stream);
+    hipError_t e = ref_usrbc(c->rg, c->rf, c->rusr, c->rsurf, c->nsurf, c->stream);
+    if (e == hipSuccess && force)
+        e = ref_mem(c->rg, c->rf, c->rusr, c->partials, c->nslots, c->hist, c->nrec++, c->step, false, c->stream);
+    return e;
+}
+
 // evs: null, or 2n events recorded around each step's bulk kernel (plain mode)
 static int step_n(lbm_ctx *c, int n, hipEvent_t *evs = nullptr) {
     if (c->stage != 0)
@@ -730,8 +742,7 @@ static int step_n(lbm_ctx *c, int n, hipEvent_t *evs = nullptr) {
             c->step++;
             const bool force = fi > 0 && c->step % fi == 0;
             HIP_TRY(c, ref_fills(c->rg, c->rf, c->rft, c->stream));
-            HIP_TRY(c, ref_usrbc(c->rg, c->rf, c->rusr, c->rsurf, c->nsurf, c->stream));
-            if (force) HIP_TRY(c, ref_mem(c->rg, c->rf, c->rusr, c->partials, c->nslots, c->hist, c->nrec++, c->step, c->stream));
+            HIP_TRY(c, ref_boundary_pass(c, force));
             HIP_TRY(c, ref_fused(c->rg, c->rf, c->rft, k == n - 1 ? c->rmac : nullptr, c->rusr, c->rprof,
                                  k == n - 1 ? c->rfpre : nullptr, c->stream));
             std::swap(c->rf, c->rft);
@@ -795,9 +806,7 @@ extern "C" int lbm_boundary(lbm_ctx *c) {
     if (rc != LBM_OK) return rc;
     const int fi = c->cfg.force_interval;
     HIP_TRY(c, ref_fills(c->rg, c->rf, c->rft, c->stream));
-    HIP_TRY(c, ref_usrbc(c->rg, c->rf, c->rusr, c->rsurf, c->nsurf, c->stream));
-    if (fi > 0 && c->step % fi == 0)
-        HIP_TRY(c, ref_mem(c->rg, c->rf, c->rusr, c->partials, c->nslots, c->hist, c->nrec++, c->step, c->stream));
+    HIP_TRY(c, ref_boundary_pass(c, fi > 0 && c->step % fi == 0));
     c->stage = 2;
     return LBM_OK;
 }

@@ -253,9 +253,13 @@ __global__ void __launch_bounds__(256) k_ref_usrbc_list(RefGeo r, double *__rest
 }
 
 // momentum exchange after usrbc: sum over fluid X, solid X+e of (f_i(X) + f_bb(i)(X+e)) e
-// (MPI Kokkos/lbm.cpp:364-392), per-block partials then a fixed-order second pass
-template <int Q>
-__global__ void __launch_bounds__(256) k_ref_mem(RefGeo r, const double *__restrict__ f, const uint8_t *__restrict__ usr,
+// (MPI Kokkos/lbm.cpp:364-392), per-block partials then a fixed-order second pass.
+// USRBC: the same pass also applies usrbc (solid lanes write, fluid lanes sum): for an interior
+// solid X+e the sum takes f_i(X-e), the value usrbc writes into f_bb(i)(X+e) (X-e is fluid:
+// the host rejects one-node fluid gaps), so no lane reads a slot another lane writes and the
+// forces are those of the two separate passes, bit for bit.
+template <int Q, bool USRBC>
+__global__ void __launch_bounds__(256) k_ref_mem(RefGeo r, double *f, const uint8_t *__restrict__ usr,
                                                  double *__restrict__ partials) {
     using L = Lat<Q>;
     const int i = blockIdx.x * blockDim.x + threadIdx.x + r.g, j = blockIdx.y + r.g;
@@ -263,12 +267,30 @@ __global__ void __launch_bounds__(256) k_ref_mem(RefGeo r, const double *__restr
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (i < r.lx - r.g) {
         const long n = (long)i + (long)r.px * ((long)j + (long)r.ly * k);
-        if (usr[n] == 1) {
+        if (USRBC && usr[n] == 0) {
             static_for<0, Q>([&](auto I) {
                 constexpr int ii = I;
                 const long nb = n + L::ex(ii) + (long)r.px * (L::ey(ii) + (long)r.ly * L::ez(ii));
+                if (usr[nb] == 1)
+                    f[RIDX(r, ii, i, j, k)] =
+                        f[RIDX(r, L::opp(ii), i + 2 * L::ex(ii), j + 2 * L::ey(ii), k + 2 * L::ez(ii))];
+            });
+        } else if (usr[n] == 1) {
+            static_for<0, Q>([&](auto I) {
+                constexpr int ii = I;
+                constexpr int ex = L::ex(ii), ey = L::ey(ii), ez = L::ez(ii);
+                const long nb = n + ex + (long)r.px * (ey + (long)r.ly * ez);
                 if (usr[nb] == 0) {
-                    const double m = f[RIDX(r, ii, i, j, k)] + f[RIDX(r, L::opp(ii), i + L::ex(ii), j + L::ey(ii), k + L::ez(ii))];
+                    double bb;
+                    if constexpr (USRBC) {
+                        const int xi = i + ex, xj = j + ey, xk = k + ez;
+                        const bool inner = xi >= r.g && xi < r.lx - r.g && xj >= r.g && xj < r.ly - r.g &&
+                                           (L::D == 2 || (xk >= r.g && xk < r.lz - r.g));
+                        bb = inner ? f[RIDX(r, ii, i - ex, j - ey, k - ez)] : f[RIDX(r, L::opp(ii), xi, xj, xk)];
+                    } else {
+                        bb = f[RIDX(r, L::opp(ii), i + ex, j + ey, k + ez)];
+                    }
+                    const double m = f[RIDX(r, ii, i, j, k)] + bb;
                     fx += m * (double)L::ex(ii);
                     fy += m * (double)L::ey(ii);
                     fz += m * (double)L::ez(ii);
@@ -495,10 +517,15 @@ hipError_t ref_usrbc(const RefGeo &r, double *f, const uint8_t *usr, const uint3
     return hipGetLastError();
 }
 
-hipError_t ref_mem(const RefGeo &r, const double *f, const uint8_t *usr, double *partials, int nblk, double *hist,
-                   int rec, long step, hipStream_t s) {
-    if (r.Q == 9) hipLaunchKernelGGL(k_ref_mem<9>, interior_grid(r), dim3(256), 0, s, r, f, usr, partials);
-    else hipLaunchKernelGGL(k_ref_mem<27>, interior_grid(r), dim3(256), 0, s, r, f, usr, partials);
+hipError_t ref_mem(const RefGeo &r, double *f, const uint8_t *usr, double *partials, int nblk, double *hist,
+                   int rec, long step, bool with_usrbc, hipStream_t s) {
+    if (with_usrbc) {
+        if (r.Q == 9) hipLaunchKernelGGL((k_ref_mem<9, true>), interior_grid(r), dim3(256), 0, s, r, f, usr, partials);
+        else hipLaunchKernelGGL((k_ref_mem<27, true>), interior_grid(r), dim3(256), 0, s, r, f, usr, partials);
+    } else {
+        if (r.Q == 9) hipLaunchKernelGGL((k_ref_mem<9, false>), interior_grid(r), dim3(256), 0, s, r, f, usr, partials);
+        else hipLaunchKernelGGL((k_ref_mem<27, false>), interior_grid(r), dim3(256), 0, s, r, f, usr, partials);
+    }
     hipLaunchKernelGGL(k_ref_reduce, dim3(1), dim3(256), 0, s, partials, nblk, hist, rec, step);
     return hipGetLastError();
 }
