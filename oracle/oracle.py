@@ -13,7 +13,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# LBM_ORACLE_LIB selects another build of the same sources (the sanitiser build, or a
+# deliberately broken copy that the pin tests must reject)
+LIB_PATH = os.environ.get("LBM_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 BODY_NONE, BODY_SPHERE, BODY_ZCYL, BODY_MASK = 0, 1, 2, 3
 INLET_UNIFORM, INLET_PARABOLIC, INLET_LEGENDRE, INLET_PROFILE = 0, 1, 2, 3
