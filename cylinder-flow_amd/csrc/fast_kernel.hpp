@@ -1,7 +1,8 @@
 // fast_kernel.hpp -- fused pull step for FP32 / FP16 / BF16 storage, V nodes per lane.
 //
 // Same boundary rules and arithmetic as the per-node path (fused_common.hpp), organised for
-// gfx950 memory throughput.  One step is three launches on the solver stream:
+// gfx950 memory throughput.  One step is one launch on the solver stream, plus a second when
+// the fix-up list below is not empty:
 //  * k_fast_v runs the packed path on every lane of every row: each lane owns V consecutive x
 //    nodes, so each of the 27 pulls and 27 stores is one V*sizeof(storage)-byte access (a
 //    dword for two FP16 nodes); the source row of a direction is a scalar and only three lane
@@ -10,20 +11,19 @@
 //    elements plus the source rows' stretches.  Optionally (aligned_pull), 16-bit interior
 //    rows load every direction aligned and build the x-1 / x+1 pairs from the neighbouring
 //    lane's dword, so a wave's request spans two cache lines instead of three.  It applies
-//    walls, periodic y/z, the inlet and the outlet, but knows nothing of the mask: solid nodes
-//    get values that no fluid node ever reads;
-//  * before it, k_link_pairs (one lane per link) or, on steps that record the force, k_link_v
-//    (one lane per node, summing the momentum exchange) copies, for every mask link of a fluid
-//    node x (direction i pulls from the solid node s = x - e_i), the stored h_opp(i)(x) into
-//    slot (s, i) of the input buffer: the packed path's plain pull then performs the half-way
-//    bounce-back itself, and the copied values give the drag/lift;
+//    walls, periodic y/z, the inlet and the outlet.  The mask's half-way bounce-back is
+//    confined to a per-row window of lanes (built once from the mask; empty for most rows):
+//    a lane inside it reads its nodes' link words (bit i: direction i pulls from a solid node)
+//    and, for each linked direction, replaces the pulled value by its own node's h_opp(i),
+//    loaded from the same row (a line the solid neighbour's lane reads anyway), and on force
+//    steps writes each node's momentum exchange to its partial slot.  The step reads only
+//    its input buffer and writes only its output buffer;
 //  * after it, k_fix_v recomputes, with the per-node pull, the few nodes the packed path still
 //    gets wrong (a compact list built once from the mask: nodes with a solid fraction, the
 //    periodic x edges, the last node of a lane that reaches into the padding, mask neighbours
-//    on the outlet column or when the stored offsets differ between opposite directions).
-// The rare boundary work thus costs the fast kernel no registers: it stays at the occupancy
-// of the packed path alone.  Both kernels use collide(), so a node's result does not depend
-// on the path it took.
+//    on the outlet column, whose clamped pull the window does not cover).
+// The rare boundary work thus costs the fast kernel few registers.  Both kernels use
+// collide(), so a node's result does not depend on the path it took.
 //  * feq_i and feq_opp(i) share w_i (3p - 1.5|u|^2 + 4.5 (e.u)^2) and differ by the sign of
 //    3 w_i e.u, so the equilibrium is evaluated per opposite pair.
 #pragma once
@@ -200,11 +200,13 @@ __device__ __forceinline__ void aligned_pull(const char *Arow, const DirTab &tab
 // the source row (or the wall bounce-back row) per direction.  The inlet lane (x = 0) and the
 // lane holding the outlet node (x = nx-1) load their row aligned and shift one value; a lane
 // past the outlet (padding) clamps the same way, so every access stays inside its row.
-template <int Q, class S, int V, bool MACROS, bool ALIGNED>
+template <int Q, class S, int V, bool FORCE, bool MACROS, bool ALIGNED>
 __device__ __forceinline__ void packed_nodes(const typename S::T *__restrict__ A, char *__restrict__ Brow,
                                              const double *__restrict__ uin, double *__restrict__ macros,
                                              const FusedArgs &a, const DirTab &tab, int y, int z, int x0,
-                                             size_t rowoff, size_t poprow) {
+                                             size_t rowoff, size_t poprow, int4 win, int tl,
+                                             const uint32_t *__restrict__ wlk, const int *__restrict__ wslot,
+                                             double *__restrict__ partials) {
     using L = Lat<Q>;
     using T = typename S::T;
     const Geo &g = a.g;
@@ -255,6 +257,51 @@ __device__ __forceinline__ void packed_nodes(const typename S::T *__restrict__ A
             bbw[i] = bb;
         });
     }
+    // Mask links (lanes of the row's window only; never the inlet or outlet lane, so element K
+    // of a pack is node K's value): direction i of a node whose source x - e_i is solid takes the
+    // node's own stored h_opp(i) (half-way bounce-back; exact as stored, because the host sends
+    // link nodes to the fix-up list when opposite directions have different storage offsets),
+    // and on force steps its momentum exchange 2 h_opp(i) e_opp(i), summed in direction order as
+    // the per-node pull does, goes to the node's partial slot.
+    if (win.y > 0 && (unsigned)tl < (unsigned)win.y) {
+        uint32_t lk[V], any = 0;
+        static_for<0, V>([&](auto K) {
+            lk[K] = wlk[(size_t)win.z + (size_t)tl * V + K];
+            any |= lk[K];
+        });
+        Pack<T, V> own[Q];
+        static_for<1, Q>([&](auto I) {   // every bounced value first, so the loads are in flight together
+            constexpr int i = I;
+            if ((any >> i) & 1u) own[i] = load_pack<T, V>(Arow + tab.dst[L::opp(i)] + b0);
+        });
+        double fx[V], fy[V], fz[V];
+        static_for<0, V>([&](auto K) { fx[K] = fy[K] = fz[K] = 0.0; });
+        static_for<1, Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i), o = L::opp(i);
+            static_for<0, V>([&](auto K) {
+                if ((lk[K] >> i) & 1u) {
+                    r[i].e[K] = own[i].e[K];
+                    if constexpr (FORCE) {
+                        const double m = 2.0 * (double)S::template decode<i, L>(own[i].e[K], a.sp.off[o]);
+                        if constexpr (ex != 0) fx[K] -= ex * m;
+                        if constexpr (ey != 0) fy[K] -= ey * m;
+                        if constexpr (ez != 0) fz[K] -= ez * m;
+                    }
+                }
+            });
+        });
+        if constexpr (FORCE) {
+            static_for<0, V>([&](auto K) {
+                const int k = wslot[(size_t)win.z + (size_t)tl * V + K];
+                if (k >= 0) {
+                    partials[3 * (size_t)k] = fx[K];
+                    partials[3 * (size_t)k + 1] = fy[K];
+                    partials[3 * (size_t)k + 2] = fz[K];
+                }
+            });
+        }
+    }
     float gv[V][Q];
     static_for<0, Q>([&](auto I) {
         constexpr int i = I;
@@ -296,82 +343,22 @@ __device__ __forceinline__ void packed_nodes(const typename S::T *__restrict__ A
 }
 
 // Every lane of every row (256-lane blocks along x, one row per (blockIdx.y, blockIdx.z)):
-// lanes past the padded row return; nothing else branches on the node, and there is no
-// barrier.
-template <int Q, class S, int V, bool MACROS, bool ALIGNED>
+// lanes past the padded row return; nothing else branches on the node but the row's link
+// window, and there is no barrier.  rowwin may be null (no mask links).
+template <int Q, class S, int V, bool FORCE, bool MACROS, bool ALIGNED>
 __global__ void __launch_bounds__(256)
     k_fast_v(const typename S::T *__restrict__ A, typename S::T *__restrict__ B, const double *__restrict__ uin,
-             double *__restrict__ macros, const FusedArgs a, const DirTab tab) {
+             double *__restrict__ macros, const int4 *__restrict__ rowwin, const uint32_t *__restrict__ wlk,
+             const int *__restrict__ wslot, double *__restrict__ partials, const FusedArgs a, const DirTab tab) {
     const Geo &g = a.g;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;   // lane index within the x row
     if (t * V >= g.nxp) return;
     const int y = blockIdx.y, z = blockIdx.z;
     const size_t row = (size_t)y + (size_t)g.ny * z;
     const size_t rowoff = (size_t)g.nxp * row, poprow = g.rs * row;
-    packed_nodes<Q, S, V, MACROS, ALIGNED>(A, (char *)(B + poprow), uin, macros, a, tab, y, z, t * V, rowoff, poprow);
-}
-
-// Pre-applied mask bounce-back, before k_fast_v on the same stream.  Half-way bounce-back
-// gives a fluid node x pulling direction i from a solid node s = x - e_i the value
-// h_opp(i)(x); slot (s, i) of A is pulled by x alone, so copying h_opp(i)(x) into it makes the
-// packed path's plain pull exact for x.  One lane per listed node (with its link word: bit i
-// set when direction i pulls from a solid node); the copy is of the stored value, exact when
-// opposite directions share their storage offset.  FORCE writes the node's momentum exchange
-// 2 h_opp(i) e_opp(i), summed in direction order as the per-node pull does, to its slot.
-template <int Q, class S, bool FORCE>
-__global__ void __launch_bounds__(256)
-    k_link_v(typename S::T *__restrict__ A, const uint32_t *__restrict__ list, const uint32_t *__restrict__ links,
-             int nlist, const int *__restrict__ slot, double *__restrict__ partials, const FusedArgs a) {
-    using L = Lat<Q>;
-    using T = typename S::T;
-    const Geo &g = a.g;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nlist) return;
-    const uint32_t node = list[j], lk = links[j];
-    const uint32_t row = node / (uint32_t)g.nxp;
-    const int x = (int)(node - row * (uint32_t)g.nxp);
-    const int y = (int)(row % (uint32_t)g.ny), z = (int)(row / (uint32_t)g.ny);
-    T val[Q];
-    static_for<1, Q>([&](auto I) {   // every bounced value first, so the loads are in flight together
-        constexpr int i = I;
-        if ((lk >> i) & 1u) val[i] = A[pel(g, L::opp(i), (uint32_t)x, row)];
-    });
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    static_for<1, Q>([&](auto I) {
-        constexpr int i = I;
-        constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i), o = L::opp(i);
-        if ((lk >> i) & 1u) {
-            // the solid source: x - e_i (the list holds no node on a periodic x edge or on the
-            // outlet column, and link bits exclude wall links), periodic wrap in y and z
-            int sy = y - ey, sz = z - ez;
-            if constexpr (ey != 0) sy = sy < 0 ? sy + g.ny : (sy >= g.ny ? sy - g.ny : sy);
-            if constexpr (ez != 0) sz = sz < 0 ? sz + g.nz : (sz >= g.nz ? sz - g.nz : sz);
-            A[pel(g, i, (uint32_t)(x - ex), (uint32_t)sy + (uint32_t)g.ny * (uint32_t)sz)] = val[i];
-            if constexpr (FORCE) {
-                const double m = 2.0 * (double)S::template decode<i, L>(val[i], a.sp.off[o]);
-                if constexpr (ex != 0) fx -= ex * m;
-                if constexpr (ey != 0) fy -= ey * m;
-                if constexpr (ez != 0) fz -= ez * m;
-            }
-        }
-    });
-    if constexpr (FORCE) {
-        const int k = slot[j];
-        partials[3 * (size_t)k] = fx;
-        partials[3 * (size_t)k + 1] = fy;
-        partials[3 * (size_t)k + 2] = fz;
-    }
-}
-
-// The copies of k_link_v without the force, one lane per link: pair j holds the element
-// indices (source, destination) of one bounced value; every destination is a distinct solid
-// slot and no source is one, so the lanes are independent.
-template <class T>
-__global__ void __launch_bounds__(256) k_link_pairs(T *__restrict__ A, const uint32_t *__restrict__ pairs, int n) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const uint2 p = reinterpret_cast<const uint2 *>(pairs)[j];
-    A[p.y] = A[p.x];
+    const int4 win = rowwin ? rowwin[row] : make_int4(0, 0, 0, 0);
+    packed_nodes<Q, S, V, FORCE, MACROS, ALIGNED>(A, (char *)(B + poprow), uin, macros, a, tab, y, z, t * V, rowoff,
+                                                  poprow, win, t - win.x, wlk, wslot, partials);
 }
 
 // The per-node recomputation of the listed nodes (padded-layout indices, row-major order),
@@ -484,8 +471,7 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// One step, three launches on s: the bounce-back copies into A's solid slots, the packed path
-// everywhere, then the listed nodes (dense mode: k_dense_v alone).
+// One step on s: the packed path everywhere, then the listed nodes (dense mode: k_dense_v alone).
 template <int Q, class S, int V, bool FORCE, bool MACROS>
 static hipError_t fast_launch(const StepBufs &b, const FusedArgs &a, hipStream_t s) {
     using T = typename S::T;
@@ -506,24 +492,16 @@ static hipError_t fast_launch(const StepBufs &b, const FusedArgs &a, hipStream_t
         if (b.ev_bulk[1] && (e = hipEventRecord(b.ev_bulk[1], s)) != hipSuccess) return e;
         return hipSuccess;
     }
-    if (b.nlink > 0 && !FORCE && b.npair > 0) {
-        hipLaunchKernelGGL((k_link_pairs<T>), dim3((unsigned)((b.npair + 255) / 256)), dim3(256), 0, s, (T *)b.A,
-                           b.lpair, b.npair);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    } else if (b.nlink > 0) {
-        hipLaunchKernelGGL((k_link_v<Q, S, FORCE>), dim3((unsigned)((b.nlink + 255) / 256)), dim3(256), 0, s,
-                           (T *)b.A, b.link, b.linklk, b.nlink, b.link_slot, b.partials, a);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
     // the aligned pull needs the whole row in one block
     const unsigned lds = b.fast_lds;
     if (b.ev_bulk[0] && (e = hipEventRecord(b.ev_bulk[0], s)) != hipSuccess) return e;
     if (sizeof(T) == 2 && V == 2 && sh.gx == 1 && b.aligned_pull)
-        hipLaunchKernelGGL((k_fast_v<Q, S, V, MACROS, (sizeof(T) == 2 && V == 2)>), dim3(sh.gx, a.g.ny, a.g.nz),
-                           dim3(sh.bx), lds, s, (const T *)b.A, (T *)b.B, b.uin, b.macros, a, tab);
+        hipLaunchKernelGGL((k_fast_v<Q, S, V, FORCE, MACROS, (sizeof(T) == 2 && V == 2)>), dim3(sh.gx, a.g.ny, a.g.nz),
+                           dim3(sh.bx), lds, s, (const T *)b.A, (T *)b.B, b.uin, b.macros, b.rowwin, b.wlk, b.wslot,
+                           b.partials, a, tab);
     else
-        hipLaunchKernelGGL((k_fast_v<Q, S, V, MACROS, false>), dim3(sh.gx, a.g.ny, a.g.nz), dim3(sh.bx), lds, s,
-                           (const T *)b.A, (T *)b.B, b.uin, b.macros, a, tab);
+        hipLaunchKernelGGL((k_fast_v<Q, S, V, FORCE, MACROS, false>), dim3(sh.gx, a.g.ny, a.g.nz), dim3(sh.bx), lds, s,
+                           (const T *)b.A, (T *)b.B, b.uin, b.macros, b.rowwin, b.wlk, b.wslot, b.partials, a, tab);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (b.ev_bulk[1] && (e = hipEventRecord(b.ev_bulk[1], s)) != hipSuccess) return e;
     if (b.nfix == 0) return hipSuccess;

@@ -62,12 +62,11 @@ struct StepBufs {
     const uint32_t *fix;        // FP32/16-bit: nodes the packed path gets wrong (padded indices)
     const uint32_t *fixlk;      // their link words: bit i set when direction i pulls from a solid node
     int nfix;
-    const uint32_t *link;       // FP32/16-bit: fluid nodes whose mask links are pre-applied by copies
-    const uint32_t *linklk;     // their link words
-    const int *link_slot;       // their force partial slots
-    int nlink;
-    const uint32_t *lpair;      // the same copies one per link: (source, destination) element index pairs
-    int npair;
+    const int4 *rowwin;         // FP32/16-bit, per row (null: no mask links anywhere): the lanes
+                                // [x, x + y) of the packed kernel hold mask links; their V nodes'
+                                // link words and force slots start at element z of wlk / wslot
+    const uint32_t *wlk;        // link words of the window nodes (0: no link, or a fix-list node)
+    const int *wslot;           // force partial slots of the window nodes (-1: none)
     const void *sfrac;          // solid fraction (FP64: double, else float) or null
     const double *uin;          // inlet velocity per (y, z)
     const int *block_slot;      // force partial slot per block (FP64) or per listed node (-1: none)
@@ -174,15 +173,13 @@ struct lbm_ctx {
     uint32_t *fix = nullptr;        // FP32/16-bit: nodes recomputed per node after the packed path
     uint32_t *fixlk = nullptr;      // their link words
     int nfix = 0;
-    uint32_t *link = nullptr;       // FP32/16-bit: nodes whose mask links are copied before the packed path
-    uint32_t *linklk = nullptr;     // their link words
-    int *link_slot = nullptr;       // their force slots
-    uint32_t *lpair = nullptr;      // one (source, destination) element index pair per link
-    int npair = 0;
-    int nlink = 0;
+    int4 *rowwin = nullptr;         // FP32/16-bit: per-row windows of mask links (see StepBufs)
+    uint32_t *wlk = nullptr;
+    int *wslot = nullptr;
+    int nlink = 0;                  // mask-neighbour nodes handled by the packed kernel
     int V = 1;                      // nodes per lane of the fused kernel
     bool aligned_pull = true;
-    bool dense = false;             // FP32/16-bit: per-node pass over every node (see StepBufs)       // see StepBufs
+    bool dense = false;             // FP32/16-bit: per-node pass over every node (see StepBufs)
     unsigned fast_lds = 0;
 
     // force history (device + host copy)
@@ -198,6 +195,15 @@ struct lbm_ctx {
 
     bool initialized = false;
     long step = 0;
+    // Fields (and, in reference mode, the populations after Update) are written on request, not
+    // every step: fields_fresh says they belong to the current state; otherwise replay_ok says the
+    // last step's input buffer is intact, so rerunning that step with the moments on rewrites the
+    // same populations and fills them in.
+    bool fields_fresh = true;
+    bool replay_ok = false;
+    // snapshot staging: device buffer (plain mode) and pinned host buffer, kept for reuse
+    double *snap_dev = nullptr, *snap_host = nullptr;
+    size_t snap_elems = 0;
     // staged step (reference mode): 0 between steps (rf holds the next step's collided
     // populations, rfpre the populations after Update), 1 after lbm_collide, 2 after
     // lbm_boundary, 3 after lbm_stream
@@ -205,3 +211,6 @@ struct lbm_ctx {
 
     SnapshotJob *snap = nullptr;
 };
+
+// materialise the fields of the last step (see lbm_capi.cpp); LBM_OK or an LBM_ERR_* code
+int ensure_fields(lbm_ctx *c);

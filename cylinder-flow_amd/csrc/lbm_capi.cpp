@@ -92,7 +92,8 @@ extern "C" const char *lbm_last_error(const lbm_ctx *ctx) { return ctx ? ctx->er
 
 static void free_all(lbm_ctx *c) {
     void *ptrs[] = {c->A, c->B, c->flags, c->dsfrac, c->duin, c->macros, c->block_slot, c->partials, c->hist, c->fix,
-                    c->fixlk, c->link, c->linklk, c->link_slot, c->lpair, c->rf, c->rft, c->rmac, c->rprof, c->rusr, c->rfpre, c->rsurf};
+                    c->fixlk, c->rowwin, c->wlk, c->wslot, c->rf, c->rft, c->rmac, c->rprof, c->rusr, c->rfpre, c->rsurf,
+                    c->snap_dev};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c->A = c->B = nullptr;
@@ -100,15 +101,19 @@ static void free_all(lbm_ctx *c) {
     c->dsfrac = nullptr;
     c->duin = c->macros = c->partials = c->hist = nullptr;
     c->block_slot = nullptr;
-    c->fix = c->fixlk = c->link = c->linklk = nullptr;
-    c->link_slot = nullptr;
-    c->lpair = nullptr;
-    c->npair = 0;
+    c->fix = c->fixlk = nullptr;
+    c->rowwin = nullptr;
+    c->wlk = nullptr;
+    c->wslot = nullptr;
     c->nfix = c->nlink = 0;
+    c->snap_dev = nullptr;
     c->rf = c->rft = c->rmac = c->rprof = c->rfpre = nullptr;
     c->rsurf = nullptr;
     c->nsurf = 0;
     c->rusr = nullptr;
+    if (c->snap_host) (void)hipHostFree(c->snap_host);
+    c->snap_host = nullptr;
+    c->snap_elems = 0;
 }
 
 extern "C" int lbm_create(const lbm_config *cfg, lbm_ctx **out) {
@@ -454,20 +459,21 @@ static int init_plain(lbm_ctx *c) {
     };
     if (c->prec == LBM_FP64) block_slots();
     // FP32/16-bit: per fluid node (row-major padded indices), either
-    //  * the link list: mask neighbours whose bounce-back is pre-applied by copying h_opp(i)
-    //    into the solid source's slot i before the packed path (exact when opposite
-    //    directions share their storage offset: FP32, or 16-bit storage with store_uref 0), or
+    //  * the link list: mask neighbours whose half-way bounce-back the packed kernel applies
+    //    in its pull, from per-row windows of link words and force slots (below), or
     //  * the fix list, recomputed per node after the packed path: nodes with a solid fraction,
     //    the periodic x edges, the last node when nx is not a multiple of the lane width V
-    //    (its lane reaches into the padding), and mask neighbours on the outlet column (whose
-    //    clamped pull shares a solid slot with its x-1 neighbour) or when copies are not exact.
+    //    (its lane reaches into the padding), and mask neighbours in the inlet or outlet lane
+    //    (whose packs the packed kernel shifts by one node) or, when opposite directions have
+    //    different storage offsets (16-bit storage with store_uref != 0), every mask neighbour
+    //    (the packed kernel substitutes stored values, not decoded ones).
     // LBM_FORCE_SLOW_PATH=1 puts every fluid node on the fix list.  Force partial slots go to
     // the forced nodes in node order across both lists, so the force sum does not depend on
     // which list a node is on.
     if (c->prec != LBM_FP64) {
         const char *force_slow = std::getenv("LBM_FORCE_SLOW_PATH");
         const bool all_slow = force_slow && force_slow[0] == '1';
-        const bool copies_exact = c->prec == LBM_FP32 || c->cfg.store_uref == 0.0;
+        const bool same_offsets = c->prec == LBM_FP32 || c->cfg.store_uref == 0.0;
         std::vector<uint32_t> fix, fixlk, link, linklk;
         std::vector<int> lslot;
         size_t fluid = 0;
@@ -480,7 +486,9 @@ static int init_plain(lbm_ctx *c) {
                     const bool f = forced(x, y, z), near = flags[node] == FL_NEAR;
                     const bool sfr = sf && c->sfrac[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] > 0.0;
                     const bool edge = (g.xper && (x == 0 || x == g.nx - 1)) || (x == g.nx - 1 && g.nx % V != 0);
-                    const bool to_fix = all_slow || sfr || edge || (near && (x == g.nx - 1 || !copies_exact));
+                    const int x0 = x / V * V;   // first node of the packed kernel's lane
+                    const bool end_lane = x0 == 0 || x0 + V >= g.nx;
+                    const bool to_fix = all_slow || sfr || edge || (near && (end_lane || !same_offsets));
                     if (!to_fix && !near) continue;
                     // link word: the per-node pull's classification (Zou-He unknowns and wall
                     // links first, outlet clamp, periodic wrap), bit i when the source is solid
@@ -539,39 +547,38 @@ static int init_plain(lbm_ctx *c) {
             HIP_TRY(c, hipMalloc(&c->fixlk, sizeof(uint32_t) * fixlk.size()));
             HIP_TRY(c, hipMemcpy(c->fixlk, fixlk.data(), sizeof(uint32_t) * fixlk.size(), hipMemcpyHostToDevice));
         }
+        // Per-row windows of the link list for the packed kernel: row r's lanes [t0, t0 + nl)
+        // cover its linked nodes; the window's nl*V nodes hold their link words and force slots
+        // (0 and -1 for nodes without links) from element `base` of wlk / wslot.
         if (c->nlink) {
-            HIP_TRY(c, hipMalloc(&c->link, sizeof(uint32_t) * link.size()));
-            HIP_TRY(c, hipMemcpy(c->link, link.data(), sizeof(uint32_t) * link.size(), hipMemcpyHostToDevice));
-            HIP_TRY(c, hipMalloc(&c->linklk, sizeof(uint32_t) * linklk.size()));
-            HIP_TRY(c, hipMemcpy(c->linklk, linklk.data(), sizeof(uint32_t) * linklk.size(), hipMemcpyHostToDevice));
-            HIP_TRY(c, hipMalloc(&c->link_slot, sizeof(int) * lslot.size()));
-            HIP_TRY(c, hipMemcpy(c->link_slot, lslot.data(), sizeof(int) * lslot.size(), hipMemcpyHostToDevice));
-        }
-        // The copies again, one entry per link (steps without force): element indices of the
-        // bounced value h_opp(i)(x) and of the solid slot (x - e_i, i), sorted by the
-        // destination, so a wave's lanes share rows.
-        if (c->nlink && (uint64_t)Q * g.plane <= UINT32_MAX) {
-            std::vector<std::pair<uint32_t, uint32_t>> pairs;   // (destination, source)
-            for (size_t j = 0; j < link.size(); j++) {
-                const uint32_t node = link[j], row = node / (uint32_t)g.nxp;
-                const int x = (int)(node - row * (uint32_t)g.nxp), y = (int)(row % (uint32_t)g.ny), z = (int)(row / (uint32_t)g.ny);
-                for (int i = 1; i < Q; i++) {
-                    if (!((linklk[j] >> i) & 1u)) continue;
-                    const int sy = (y - e[3 * i + 1] + g.ny) % g.ny, sz = (z - e[3 * i + 2] + g.nz) % g.nz;
-                    const size_t src = pel(g, i, (uint32_t)(x - e[3 * i]), (uint32_t)(sy + g.ny * sz));
-                    pairs.emplace_back((uint32_t)src, (uint32_t)pel(g, opp[i], (uint32_t)x, row));
+            std::vector<int4> win((size_t)g.ny * g.nz, make_int4(0, 0, 0, 0));
+            std::vector<uint32_t> wlk;
+            std::vector<int> wslot;
+            for (size_t j = 0; j < link.size();) {
+                const uint32_t row = link[j] / (uint32_t)g.nxp;
+                size_t k = j;
+                while (k < link.size() && link[k] / (uint32_t)g.nxp == row) k++;   // the row's entries (node order)
+                const int t0 = (int)(link[j] - row * (uint32_t)g.nxp) / V;
+                const int t1 = (int)(link[k - 1] - row * (uint32_t)g.nxp) / V;
+                const size_t base = wlk.size();
+                if (base + (size_t)(t1 - t0 + 1) * V > (size_t)INT32_MAX)
+                    return set_err(c, LBM_ERR_INVALID, "too many mask links");
+                wlk.resize(base + (size_t)(t1 - t0 + 1) * V, 0u);
+                wslot.resize(wlk.size(), -1);
+                for (size_t m = j; m < k; m++) {
+                    const size_t at = base + (link[m] - row * (uint32_t)g.nxp) - (size_t)t0 * V;
+                    wlk[at] = linklk[m];
+                    wslot[at] = lslot[m];
                 }
+                win[row] = make_int4(t0, t1 - t0 + 1, (int)base, 0);
+                j = k;
             }
-            std::sort(pairs.begin(), pairs.end());
-            if (pairs.size() > (size_t)INT32_MAX) return set_err(c, LBM_ERR_INVALID, "too many mask links");
-            std::vector<uint32_t> flat(2 * pairs.size());
-            for (size_t k = 0; k < pairs.size(); k++) {
-                flat[2 * k] = pairs[k].second;
-                flat[2 * k + 1] = pairs[k].first;
-            }
-            c->npair = (int)pairs.size();
-            HIP_TRY(c, hipMalloc(&c->lpair, sizeof(uint32_t) * flat.size()));
-            HIP_TRY(c, hipMemcpy(c->lpair, flat.data(), sizeof(uint32_t) * flat.size(), hipMemcpyHostToDevice));
+            HIP_TRY(c, hipMalloc(&c->rowwin, sizeof(int4) * win.size()));
+            HIP_TRY(c, hipMemcpy(c->rowwin, win.data(), sizeof(int4) * win.size(), hipMemcpyHostToDevice));
+            HIP_TRY(c, hipMalloc(&c->wlk, sizeof(uint32_t) * wlk.size()));
+            HIP_TRY(c, hipMemcpy(c->wlk, wlk.data(), sizeof(uint32_t) * wlk.size(), hipMemcpyHostToDevice));
+            HIP_TRY(c, hipMalloc(&c->wslot, sizeof(int) * wslot.size()));
+            HIP_TRY(c, hipMemcpy(c->wslot, wslot.data(), sizeof(int) * wslot.size(), hipMemcpyHostToDevice));
         }
     }
     const size_t esz = c->prec == LBM_FP64 ? 8 : c->prec == LBM_FP32 ? 4 : 2;
@@ -688,6 +695,8 @@ extern "C" int lbm_initialize(lbm_ctx *c) {
     c->step = 0;
     c->stage = 0;
     c->nrec = c->nrec_checked = 0;
+    c->fields_fresh = true;   // the initial macros
+    c->replay_ok = false;
     return LBM_OK;
 }
 
@@ -699,12 +708,9 @@ StepBufs step_bufs(const lbm_ctx *c, const void *A, void *B) {
     b.fix = c->fix;
     b.fixlk = c->fixlk;
     b.nfix = c->nfix;
-    b.link = c->link;
-    b.linklk = c->linklk;
-    b.link_slot = c->link_slot;
-    b.lpair = c->lpair;
-    b.npair = c->npair;
-    b.nlink = c->nlink;
+    b.rowwin = c->rowwin;
+    b.wlk = c->wlk;
+    b.wslot = c->wslot;
     b.sfrac = c->dsfrac;
     b.uin = c->duin;
     b.block_slot = c->block_slot;
@@ -743,9 +749,10 @@ static int step_n(lbm_ctx *c, int n, hipEvent_t *evs = nullptr) {
             const bool force = fi > 0 && c->step % fi == 0;
             HIP_TRY(c, ref_fills(c->rg, c->rf, c->rft, c->stream));
             HIP_TRY(c, ref_boundary_pass(c, force));
-            HIP_TRY(c, ref_fused(c->rg, c->rf, c->rft, k == n - 1 ? c->rmac : nullptr, c->rusr, c->rprof,
-                                 k == n - 1 ? c->rfpre : nullptr, c->stream));
+            HIP_TRY(c, ref_fused(c->rg, c->rf, c->rft, nullptr, c->rusr, c->rprof, nullptr, c->stream));
             std::swap(c->rf, c->rft);
+            c->fields_fresh = false;
+            c->replay_ok = true;
         }
         return LBM_OK;
     }
@@ -758,10 +765,30 @@ static int step_n(lbm_ctx *c, int n, hipEvent_t *evs = nullptr) {
             b.ev_bulk[0] = evs[2 * k];
             b.ev_bulk[1] = evs[2 * k + 1];
         }
-        HIP_TRY(c, plain_step(c->Q, c->prec, c->V, b, force, k == n - 1, a, c->stream));
+        HIP_TRY(c, plain_step(c->Q, c->prec, c->V, b, force, false, a, c->stream));
         if (force) HIP_TRY(c, force_reduce(c->partials, c->nslots, c->partials + 3 * (size_t)c->nslots, c->hist, c->nrec++, c->step, c->stream));
         std::swap(c->A, c->B);
+        c->fields_fresh = false;
+        c->replay_ok = true;
     }
+    return LBM_OK;
+}
+
+// The moments of the last step (and, in reference mode, the populations after its Update) are
+// computed on request: the steps write only populations, and the last step is run again from
+// its input buffer, which no later work has touched, with the moments on.  The populations it
+// writes are those already in the output buffer, bit for bit (same kernel, same input), so the
+// state does not change; the cost is one step, paid only when fields are read.
+int ensure_fields(lbm_ctx *c) {
+    if (c->fields_fresh) return LBM_OK;
+    if (!c->replay_ok) return set_err(c, LBM_ERR_STATE, "fields of the current state are not available");
+    if (ref_mode(c->cfg.bc_mode)) {
+        HIP_TRY(c, ref_fused(c->rg, c->rft, c->rf, c->rmac, c->rusr, c->rprof, c->rfpre, c->stream));
+    } else {
+        const FusedArgs a = ctx_fused_args(c);
+        HIP_TRY(c, plain_step(c->Q, c->prec, c->V, step_bufs(c, c->B, c->A), false, true, a, c->stream));
+    }
+    c->fields_fresh = true;
     return LBM_OK;
 }
 
@@ -794,6 +821,11 @@ static int staged(lbm_ctx *c, int want, const char *name) {
 extern "C" int lbm_collide(lbm_ctx *c) {
     int rc = staged(c, 0, "lbm_collide");
     if (rc != LBM_OK) return rc;
+    // the staged fills write the ghost slots of both buffers, so the last fused step cannot be
+    // rerun once the step is open: materialise its fields first
+    rc = ensure_fields(c);
+    if (rc != LBM_OK) return rc;
+    c->replay_ok = false;
     rc = grow_hist(c, c->nrec + 2);
     if (rc != LBM_OK) return rc;
     c->step++;
@@ -825,6 +857,8 @@ extern "C" int lbm_update(lbm_ctx *c) {
     if (rc != LBM_OK) return rc;
     HIP_TRY(c, ref_update(c->rg, c->rf, c->rmac, c->rusr, c->rprof, c->rfpre, c->stream));
     c->stage = 0;
+    c->fields_fresh = true;   // ref_update writes the moments and the populations after Update
+    c->replay_ok = false;     // (it works in place: the fused kernel's input is gone)
     return LBM_OK;
 }
 
@@ -919,6 +953,10 @@ extern "C" int lbm_get_fields(lbm_ctx *c, double *p, double *u, double *v, doubl
     if (!c) return LBM_ERR_INVALID;
     if (!c->initialized) return set_err(c, LBM_ERR_STATE, "lbm_get_fields before lbm_initialize");
     double *outs[4] = {p, u, v, w};
+    if (c->stage == 0) {
+        const int rc = ensure_fields(c);
+        if (rc != LBM_OK) return rc;
+    }
     if (ref_mode(c->cfg.bc_mode)) {
         std::vector<double> pad(c->rg.plane);
         for (int k = 0; k < 4; k++) {
@@ -952,6 +990,10 @@ extern "C" int lbm_get_populations(lbm_ctx *c, double *f) {
     if (ref_mode(c->cfg.bc_mode)) {
         // between steps the reference's f is the state after Update; inside a staged step, the
         // state after the last stage
+        if (c->stage == 0) {
+            const int rc = ensure_fields(c);
+            if (rc != LBM_OK) return rc;
+        }
         const double *src = c->stage == 0 ? c->rfpre : c->rf;
         const RefGeo &r = c->rg;
         std::vector<double> pad(r.plane);   // one direction in the node layout
@@ -985,6 +1027,10 @@ extern "C" int lbm_set_populations(lbm_ctx *c, const double *f) {
     if (!c->initialized) return set_err(c, LBM_ERR_STATE, "lbm_set_populations before lbm_initialize");
     if (ref_mode(c->cfg.bc_mode))
         return set_err(c, LBM_ERR_INVALID, "reference mode keeps ghost state; populations cannot be replaced");
+    // the fields stay those of the last step (materialised now, while its input still exists)
+    const int rc = ensure_fields(c);
+    if (rc != LBM_OK) return rc;
+    c->replay_ok = false;
     const FusedArgs a = ctx_fused_args(c);
     double *tmp = nullptr;
     HIP_TRY(c, hipMalloc(&tmp, sizeof(double) * c->N));

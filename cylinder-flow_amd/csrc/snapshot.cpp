@@ -157,25 +157,39 @@ extern "C" int lbm_snapshot_async(lbm_ctx *c, const char *path, int n) {
     }
     int rc = snapshot_wait(c);
     if (rc != LBM_OK) return rc;
+    rc = ensure_fields(c);
+    if (rc != LBM_OK) return rc;
     const bool ref = c->cfg.bc_mode != LBM_BC_PLAIN;
     const size_t plane = ref ? c->rg.plane : c->N;
-    double *pinned = nullptr, *dtmp = nullptr;
     hipEvent_t ev = nullptr;
     auto fail = [&](hipError_t e, const char *what) {
-        if (pinned) (void)hipHostFree(pinned);
-        if (dtmp) (void)hipFree(dtmp);
         if (ev) (void)hipEventDestroy(ev);
         c->err = std::string(what) + ": " + hipGetErrorString(e);
         return LBM_ERR_HIP;
     };
-    hipError_t e = hipHostMalloc((void **)&pinned, sizeof(double) * 4 * plane, hipHostMallocDefault);
-    if (e != hipSuccess) return fail(e, "hipHostMalloc");
+    // staging buffers kept in the context (freed by lbm_destroy): the worker thread only reads
+    // the pinned copy, and the next snapshot joins it before reusing them
+    if (c->snap_elems < 4 * plane) {
+        if (c->snap_host) (void)hipHostFree(c->snap_host);
+        if (c->snap_dev) (void)hipFree(c->snap_dev);
+        c->snap_host = c->snap_dev = nullptr;
+        c->snap_elems = 0;
+        hipError_t e = hipHostMalloc((void **)&c->snap_host, sizeof(double) * 4 * plane, hipHostMallocDefault);
+        if (e != hipSuccess) return fail(e, "hipHostMalloc");
+        if (!ref) {
+            e = hipMalloc(&c->snap_dev, sizeof(double) * 4 * plane);
+            if (e != hipSuccess) return fail(e, "hipMalloc");
+        }
+        c->snap_elems = 4 * plane;
+    }
+    double *const pinned = c->snap_host;
+    hipError_t e;
     if (ref) {
         e = hipMemcpyAsync(pinned, c->rmac, sizeof(double) * 4 * plane, hipMemcpyDeviceToHost, c->stream);
         if (e != hipSuccess) return fail(e, "hipMemcpyAsync");
     } else {
-        e = hipMalloc(&dtmp, sizeof(double) * 4 * plane);
-        if (e != hipSuccess) return fail(e, "hipMalloc");
+        double *dtmp = c->snap_dev;
+        e = hipSuccess;
         for (int k = 0; k < 4 && e == hipSuccess; k++) e = macros_extract(c->macros, c->flags, dtmp + k * plane, k, c->geo, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(pinned, dtmp, sizeof(double) * 4 * plane, hipMemcpyDeviceToHost, c->stream);
         if (e != hipSuccess) return fail(e, "snapshot copy");
@@ -189,14 +203,12 @@ extern "C" int lbm_snapshot_async(lbm_ctx *c, const char *path, int n) {
     const int D = c->D, glx = c->geo.nx, gly = c->geo.ny, glz = c->geo.nz;
     const RefGeo rg = c->rg;
     const std::string file(path);
-    job->th = std::thread([job, pinned, dtmp, ev, ref, plane, D, glx, gly, glz, rg, file, n]() {
+    job->th = std::thread([job, pinned, ev, ref, plane, D, glx, gly, glz, rg, file, n]() {
         hipError_t e2 = hipEventSynchronize(ev);
         (void)hipEventDestroy(ev);
-        if (dtmp) (void)hipFree(dtmp);
         if (e2 != hipSuccess) {
             job->rc = LBM_ERR_HIP;
             job->err = std::string("snapshot: ") + hipGetErrorString(e2);
-            (void)hipHostFree(pinned);
             return;
         }
         const size_t N = (size_t)glx * gly * glz;
@@ -218,7 +230,6 @@ extern "C" int lbm_snapshot_async(lbm_ctx *c, const char *path, int n) {
         }
         // macros are stored p, u, v, w
         job->rc = write_plt(file.c_str(), D, glx, gly, glz, n, fld[1], fld[2], fld[3], fld[0], job->err);
-        (void)hipHostFree(pinned);
     });
     return LBM_OK;
 }
