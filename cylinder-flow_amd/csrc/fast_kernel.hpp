@@ -15,9 +15,11 @@
 //    confined to a per-row window of lanes (built once from the mask; empty for most rows):
 //    a lane inside it reads its nodes' link words (bit i: direction i pulls from a solid node)
 //    and, for each linked direction, replaces the pulled value by its own node's h_opp(i),
-//    loaded from the same row (a line the solid neighbour's lane reads anyway), and on force
-//    steps writes each node's momentum exchange to its partial slot.  The step reads only
-//    its input buffer and writes only its output buffer;
+//    loaded from the same row (a line the solid neighbour's lane reads anyway).  The step
+//    reads only its input buffer and writes only its output buffer;
+//  * on steps that record the force, k_link_force (one lane per linked node, before it) sums
+//    each linked node's momentum exchange from the same input values into its partial slot:
+//    the FP64 sums stay out of the packed kernel, which keeps one instruction schedule;
 //  * after it, k_fix_v recomputes, with the per-node pull, the few nodes the packed path still
 //    gets wrong (a compact list built once from the mask: nodes with a solid fraction, the
 //    periodic x edges, the last node of a lane that reaches into the padding, mask neighbours
@@ -200,13 +202,12 @@ __device__ __forceinline__ void aligned_pull(const char *Arow, const DirTab &tab
 // the source row (or the wall bounce-back row) per direction.  The inlet lane (x = 0) and the
 // lane holding the outlet node (x = nx-1) load their row aligned and shift one value; a lane
 // past the outlet (padding) clamps the same way, so every access stays inside its row.
-template <int Q, class S, int V, bool FORCE, bool MACROS, bool ALIGNED>
+template <int Q, class S, int V, bool MACROS, bool ALIGNED>
 __device__ __forceinline__ void packed_nodes(const typename S::T *__restrict__ A, char *__restrict__ Brow,
                                              const double *__restrict__ uin, double *__restrict__ macros,
                                              const FusedArgs &a, const DirTab &tab, int y, int z, int x0,
                                              size_t rowoff, size_t poprow, int4 win, int tl,
-                                             const uint32_t *__restrict__ wlk, const int *__restrict__ wslot,
-                                             double *__restrict__ partials) {
+                                             const uint32_t *__restrict__ wlk) {
     using L = Lat<Q>;
     using T = typename S::T;
     const Geo &g = a.g;
@@ -260,9 +261,7 @@ __device__ __forceinline__ void packed_nodes(const typename S::T *__restrict__ A
     // Mask links (lanes of the row's window only; never the inlet or outlet lane, so element K
     // of a pack is node K's value): direction i of a node whose source x - e_i is solid takes the
     // node's own stored h_opp(i) (half-way bounce-back; exact as stored, because the host sends
-    // link nodes to the fix-up list when opposite directions have different storage offsets),
-    // and on force steps its momentum exchange 2 h_opp(i) e_opp(i), summed in direction order as
-    // the per-node pull does, goes to the node's partial slot.
+    // link nodes to the fix-up list when opposite directions have different storage offsets).
     if (win.y > 0 && (unsigned)tl < (unsigned)win.y) {
         uint32_t lk[V], any = 0;
         static_for<0, V>([&](auto K) {
@@ -274,33 +273,12 @@ __device__ __forceinline__ void packed_nodes(const typename S::T *__restrict__ A
             constexpr int i = I;
             if ((any >> i) & 1u) own[i] = load_pack<T, V>(Arow + tab.dst[L::opp(i)] + b0);
         });
-        double fx[V], fy[V], fz[V];
-        static_for<0, V>([&](auto K) { fx[K] = fy[K] = fz[K] = 0.0; });
         static_for<1, Q>([&](auto I) {
             constexpr int i = I;
-            constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i), o = L::opp(i);
             static_for<0, V>([&](auto K) {
-                if ((lk[K] >> i) & 1u) {
-                    r[i].e[K] = own[i].e[K];
-                    if constexpr (FORCE) {
-                        const double m = 2.0 * (double)S::template decode<i, L>(own[i].e[K], a.sp.off[o]);
-                        if constexpr (ex != 0) fx[K] -= ex * m;
-                        if constexpr (ey != 0) fy[K] -= ey * m;
-                        if constexpr (ez != 0) fz[K] -= ez * m;
-                    }
-                }
+                if ((lk[K] >> i) & 1u) r[i].e[K] = own[i].e[K];
             });
         });
-        if constexpr (FORCE) {
-            static_for<0, V>([&](auto K) {
-                const int k = wslot[(size_t)win.z + (size_t)tl * V + K];
-                if (k >= 0) {
-                    partials[3 * (size_t)k] = fx[K];
-                    partials[3 * (size_t)k + 1] = fy[K];
-                    partials[3 * (size_t)k + 2] = fz[K];
-                }
-            });
-        }
     }
     float gv[V][Q];
     static_for<0, Q>([&](auto I) {
@@ -345,11 +323,11 @@ __device__ __forceinline__ void packed_nodes(const typename S::T *__restrict__ A
 // Every lane of every row (256-lane blocks along x, one row per (blockIdx.y, blockIdx.z)):
 // lanes past the padded row return; nothing else branches on the node but the row's link
 // window, and there is no barrier.  rowwin may be null (no mask links).
-template <int Q, class S, int V, bool FORCE, bool MACROS, bool ALIGNED>
+template <int Q, class S, int V, bool MACROS, bool ALIGNED>
 __global__ void __launch_bounds__(256)
     k_fast_v(const typename S::T *__restrict__ A, typename S::T *__restrict__ B, const double *__restrict__ uin,
              double *__restrict__ macros, const int4 *__restrict__ rowwin, const uint32_t *__restrict__ wlk,
-             const int *__restrict__ wslot, double *__restrict__ partials, const FusedArgs a, const DirTab tab) {
+             const FusedArgs a, const DirTab tab) {
     const Geo &g = a.g;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;   // lane index within the x row
     if (t * V >= g.nxp) return;
@@ -357,8 +335,47 @@ __global__ void __launch_bounds__(256)
     const size_t row = (size_t)y + (size_t)g.ny * z;
     const size_t rowoff = (size_t)g.nxp * row, poprow = g.rs * row;
     const int4 win = rowwin ? rowwin[row] : make_int4(0, 0, 0, 0);
-    packed_nodes<Q, S, V, FORCE, MACROS, ALIGNED>(A, (char *)(B + poprow), uin, macros, a, tab, y, z, t * V, rowoff,
-                                                  poprow, win, t - win.x, wlk, wslot, partials);
+    packed_nodes<Q, S, V, MACROS, ALIGNED>(A, (char *)(B + poprow), uin, macros, a, tab, y, z, t * V, rowoff, poprow,
+                                           win, t - win.x, wlk);
+}
+
+// Momentum exchange of the linked nodes on force steps, before k_fast_v on the same stream and
+// from the same input buffer (neither writes it): one lane per linked node (padded index and
+// link word), each bounced value 2 h_opp(i)(x) e_opp(i) summed in direction order, as the
+// per-node pull does, into the node's partial slot.
+template <int Q, class S>
+__global__ void __launch_bounds__(256)
+    k_link_force(const typename S::T *__restrict__ A, const uint32_t *__restrict__ list,
+                 const uint32_t *__restrict__ links, const int *__restrict__ slot, int nlist,
+                 double *__restrict__ partials, const FusedArgs a) {
+    using L = Lat<Q>;
+    using T = typename S::T;
+    const Geo &g = a.g;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nlist) return;
+    const uint32_t node = list[j], lk = links[j];
+    const uint32_t row = node / (uint32_t)g.nxp;
+    const uint32_t x = node - row * (uint32_t)g.nxp;
+    T val[Q];
+    static_for<1, Q>([&](auto I) {   // every bounced value first, so the loads are in flight together
+        constexpr int i = I;
+        if ((lk >> i) & 1u) val[i] = A[pel(g, L::opp(i), x, row)];
+    });
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    static_for<1, Q>([&](auto I) {
+        constexpr int i = I;
+        constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i), o = L::opp(i);
+        if ((lk >> i) & 1u) {
+            const double m = 2.0 * (double)S::template decode<i, L>(val[i], a.sp.off[o]);
+            if constexpr (ex != 0) fx -= ex * m;
+            if constexpr (ey != 0) fy -= ey * m;
+            if constexpr (ez != 0) fz -= ez * m;
+        }
+    });
+    const int k = slot[j];
+    partials[3 * (size_t)k] = fx;
+    partials[3 * (size_t)k + 1] = fy;
+    partials[3 * (size_t)k + 2] = fz;
 }
 
 // The per-node recomputation of the listed nodes (padded-layout indices, row-major order),
@@ -492,16 +509,20 @@ static hipError_t fast_launch(const StepBufs &b, const FusedArgs &a, hipStream_t
         if (b.ev_bulk[1] && (e = hipEventRecord(b.ev_bulk[1], s)) != hipSuccess) return e;
         return hipSuccess;
     }
+    if (FORCE && b.nlink > 0) {
+        hipLaunchKernelGGL((k_link_force<Q, S>), dim3((unsigned)((b.nlink + 255) / 256)), dim3(256), 0, s, (const T *)b.A,
+                           b.link, b.linklk, b.link_slot, b.nlink, b.partials, a);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     // the aligned pull needs the whole row in one block
     const unsigned lds = b.fast_lds;
     if (b.ev_bulk[0] && (e = hipEventRecord(b.ev_bulk[0], s)) != hipSuccess) return e;
     if (sizeof(T) == 2 && V == 2 && sh.gx == 1 && b.aligned_pull)
-        hipLaunchKernelGGL((k_fast_v<Q, S, V, FORCE, MACROS, (sizeof(T) == 2 && V == 2)>), dim3(sh.gx, a.g.ny, a.g.nz),
-                           dim3(sh.bx), lds, s, (const T *)b.A, (T *)b.B, b.uin, b.macros, b.rowwin, b.wlk, b.wslot,
-                           b.partials, a, tab);
+        hipLaunchKernelGGL((k_fast_v<Q, S, V, MACROS, (sizeof(T) == 2 && V == 2)>), dim3(sh.gx, a.g.ny, a.g.nz),
+                           dim3(sh.bx), lds, s, (const T *)b.A, (T *)b.B, b.uin, b.macros, b.rowwin, b.wlk, a, tab);
     else
-        hipLaunchKernelGGL((k_fast_v<Q, S, V, FORCE, MACROS, false>), dim3(sh.gx, a.g.ny, a.g.nz), dim3(sh.bx), lds, s,
-                           (const T *)b.A, (T *)b.B, b.uin, b.macros, b.rowwin, b.wlk, b.wslot, b.partials, a, tab);
+        hipLaunchKernelGGL((k_fast_v<Q, S, V, MACROS, false>), dim3(sh.gx, a.g.ny, a.g.nz), dim3(sh.bx), lds, s,
+                           (const T *)b.A, (T *)b.B, b.uin, b.macros, b.rowwin, b.wlk, a, tab);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (b.ev_bulk[1] && (e = hipEventRecord(b.ev_bulk[1], s)) != hipSuccess) return e;
     if (b.nfix == 0) return hipSuccess;

@@ -64,9 +64,12 @@ struct StepBufs {
     int nfix;
     const int4 *rowwin;         // FP32/16-bit, per row (null: no mask links anywhere): the lanes
                                 // [x, x + y) of the packed kernel hold mask links; their V nodes'
-                                // link words and force slots start at element z of wlk / wslot
+                                // link words start at element z of wlk
     const uint32_t *wlk;        // link words of the window nodes (0: no link, or a fix-list node)
-    const int *wslot;           // force partial slots of the window nodes (-1: none)
+    const uint32_t *link;       // the linked nodes the packed kernel handles (padded indices) ...
+    const uint32_t *linklk;     // ... their link words
+    const int *link_slot;       // ... and force partial slots (force steps: k_link_force)
+    int nlink;
     const void *sfrac;          // solid fraction (FP64: double, else float) or null
     const double *uin;          // inlet velocity per (y, z)
     const int *block_slot;      // force partial slot per block (FP64) or per listed node (-1: none)
@@ -93,6 +96,10 @@ hipError_t plain_decode(int Q, int prec, const void *A, double *out, const uint8
                         const FusedArgs &a, hipStream_t s);
 hipError_t plain_encode(int Q, int prec, void *A, const double *in, const uint8_t *flags, int dir,
                         const FusedArgs &a, hipStream_t s);
+// number of non-finite values of one direction over a box of nodes (see k_count_nonfinite) into
+// *bad (device memory), for storage format prec
+hipError_t count_nonfinite(int prec, const void *base, size_t sy, size_t sz, int nx, int ny, int nz,
+                           const uint8_t *flags, int fpitch, unsigned *bad, hipStream_t s);
 // moment plane k (p, u, v, w) at the physical nodes; 0 at solid nodes
 hipError_t macros_extract(const double *macros, const uint8_t *flags, double *out, int k, const Geo &g, hipStream_t s);
 
@@ -175,12 +182,16 @@ struct lbm_ctx {
     int nfix = 0;
     int4 *rowwin = nullptr;         // FP32/16-bit: per-row windows of mask links (see StepBufs)
     uint32_t *wlk = nullptr;
-    int *wslot = nullptr;
-    int nlink = 0;                  // mask-neighbour nodes handled by the packed kernel
+    uint32_t *link = nullptr;       // mask-neighbour nodes handled by the packed kernel, their link
+    uint32_t *linklk = nullptr;     // words and force slots (for k_link_force)
+    int *link_slot = nullptr;
+    int nlink = 0;
     int V = 1;                      // nodes per lane of the fused kernel
     bool aligned_pull = true;
     bool dense = false;             // FP32/16-bit: per-node pass over every node (see StepBufs)
     unsigned fast_lds = 0;
+
+    unsigned *dbad = nullptr;       // lbm_sync's count of non-finite populations (device)
 
     // force history (device + host copy)
     double *hist = nullptr;

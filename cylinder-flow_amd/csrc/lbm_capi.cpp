@@ -92,8 +92,8 @@ extern "C" const char *lbm_last_error(const lbm_ctx *ctx) { return ctx ? ctx->er
 
 static void free_all(lbm_ctx *c) {
     void *ptrs[] = {c->A, c->B, c->flags, c->dsfrac, c->duin, c->macros, c->block_slot, c->partials, c->hist, c->fix,
-                    c->fixlk, c->rowwin, c->wlk, c->wslot, c->rf, c->rft, c->rmac, c->rprof, c->rusr, c->rfpre, c->rsurf,
-                    c->snap_dev};
+                    c->fixlk, c->rowwin, c->wlk, c->link, c->linklk, c->link_slot, c->rf, c->rft, c->rmac, c->rprof, c->rusr, c->rfpre, c->rsurf,
+                    c->snap_dev, c->dbad};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c->A = c->B = nullptr;
@@ -104,9 +104,11 @@ static void free_all(lbm_ctx *c) {
     c->fix = c->fixlk = nullptr;
     c->rowwin = nullptr;
     c->wlk = nullptr;
-    c->wslot = nullptr;
+    c->link = c->linklk = nullptr;
+    c->link_slot = nullptr;
     c->nfix = c->nlink = 0;
     c->snap_dev = nullptr;
+    c->dbad = nullptr;
     c->rf = c->rft = c->rmac = c->rprof = c->rfpre = nullptr;
     c->rsurf = nullptr;
     c->nsurf = 0;
@@ -548,12 +550,11 @@ static int init_plain(lbm_ctx *c) {
             HIP_TRY(c, hipMemcpy(c->fixlk, fixlk.data(), sizeof(uint32_t) * fixlk.size(), hipMemcpyHostToDevice));
         }
         // Per-row windows of the link list for the packed kernel: row r's lanes [t0, t0 + nl)
-        // cover its linked nodes; the window's nl*V nodes hold their link words and force slots
-        // (0 and -1 for nodes without links) from element `base` of wlk / wslot.
+        // cover its linked nodes; the window's nl*V nodes hold their link words (0 for nodes
+        // without links) from element `base` of wlk.  The list itself serves force steps.
         if (c->nlink) {
             std::vector<int4> win((size_t)g.ny * g.nz, make_int4(0, 0, 0, 0));
             std::vector<uint32_t> wlk;
-            std::vector<int> wslot;
             for (size_t j = 0; j < link.size();) {
                 const uint32_t row = link[j] / (uint32_t)g.nxp;
                 size_t k = j;
@@ -564,12 +565,8 @@ static int init_plain(lbm_ctx *c) {
                 if (base + (size_t)(t1 - t0 + 1) * V > (size_t)INT32_MAX)
                     return set_err(c, LBM_ERR_INVALID, "too many mask links");
                 wlk.resize(base + (size_t)(t1 - t0 + 1) * V, 0u);
-                wslot.resize(wlk.size(), -1);
-                for (size_t m = j; m < k; m++) {
-                    const size_t at = base + (link[m] - row * (uint32_t)g.nxp) - (size_t)t0 * V;
-                    wlk[at] = linklk[m];
-                    wslot[at] = lslot[m];
-                }
+                for (size_t m = j; m < k; m++)
+                    wlk[base + (link[m] - row * (uint32_t)g.nxp) - (size_t)t0 * V] = linklk[m];
                 win[row] = make_int4(t0, t1 - t0 + 1, (int)base, 0);
                 j = k;
             }
@@ -577,8 +574,12 @@ static int init_plain(lbm_ctx *c) {
             HIP_TRY(c, hipMemcpy(c->rowwin, win.data(), sizeof(int4) * win.size(), hipMemcpyHostToDevice));
             HIP_TRY(c, hipMalloc(&c->wlk, sizeof(uint32_t) * wlk.size()));
             HIP_TRY(c, hipMemcpy(c->wlk, wlk.data(), sizeof(uint32_t) * wlk.size(), hipMemcpyHostToDevice));
-            HIP_TRY(c, hipMalloc(&c->wslot, sizeof(int) * wslot.size()));
-            HIP_TRY(c, hipMemcpy(c->wslot, wslot.data(), sizeof(int) * wslot.size(), hipMemcpyHostToDevice));
+            HIP_TRY(c, hipMalloc(&c->link, sizeof(uint32_t) * link.size()));
+            HIP_TRY(c, hipMemcpy(c->link, link.data(), sizeof(uint32_t) * link.size(), hipMemcpyHostToDevice));
+            HIP_TRY(c, hipMalloc(&c->linklk, sizeof(uint32_t) * linklk.size()));
+            HIP_TRY(c, hipMemcpy(c->linklk, linklk.data(), sizeof(uint32_t) * linklk.size(), hipMemcpyHostToDevice));
+            HIP_TRY(c, hipMalloc(&c->link_slot, sizeof(int) * lslot.size()));
+            HIP_TRY(c, hipMemcpy(c->link_slot, lslot.data(), sizeof(int) * lslot.size(), hipMemcpyHostToDevice));
         }
     }
     const size_t esz = c->prec == LBM_FP64 ? 8 : c->prec == LBM_FP32 ? 4 : 2;
@@ -687,6 +688,10 @@ extern "C" int lbm_initialize(lbm_ctx *c) {
     if (!c) return LBM_ERR_INVALID;
     if (c->initialized) return set_err(c, LBM_ERR_STATE, "already initialised");
     int rc = c->cfg.bc_mode == LBM_BC_PLAIN ? init_plain(c) : init_ref(c);
+    if (rc == LBM_OK) {
+        const hipError_t e = hipMalloc(&c->dbad, sizeof(unsigned));
+        if (e != hipSuccess) rc = set_err(c, LBM_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
     if (rc != LBM_OK) {
         free_all(c);
         return rc;
@@ -710,7 +715,10 @@ StepBufs step_bufs(const lbm_ctx *c, const void *A, void *B) {
     b.nfix = c->nfix;
     b.rowwin = c->rowwin;
     b.wlk = c->wlk;
-    b.wslot = c->wslot;
+    b.link = c->link;
+    b.linklk = c->linklk;
+    b.link_slot = c->link_slot;
+    b.nlink = c->nlink;
     b.sfrac = c->dsfrac;
     b.uin = c->duin;
     b.block_slot = c->block_slot;
@@ -935,6 +943,36 @@ extern "C" int lbm_sync(lbm_ctx *c) {
                                                           std::to_string((long)h[4 * (size_t)r]) + ") is not finite or exceeds force_limit");
             }
     }
+    // the state itself, whether or not forces are recorded: the rest population of every live
+    // node (a non-finite value reaches it within a step through the moments)
+    if (c->initialized) {
+        hipError_t e;
+        if (ref_mode(c->cfg.bc_mode)) {
+            const RefGeo &r = c->rg;
+            const int gz = c->D == 3 ? r.g : 0;
+            e = count_nonfinite(LBM_FP64, c->rf + r.g + r.rs * ((size_t)r.g + (size_t)r.ly * gz), r.rs, r.rs * r.ly, r.glx,
+                                r.gly, r.glz, nullptr, 0, c->dbad, c->stream);
+        } else {
+            e = count_nonfinite(c->prec, c->A, c->geo.rs, c->geo.rs * c->geo.ny, c->geo.nx, c->geo.ny, c->geo.nz, c->flags,
+                                c->geo.nxp, c->dbad, c->stream);
+        }
+        unsigned bad = 0;
+        if (e == hipSuccess) e = hipMemcpyAsync(&bad, c->dbad, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return set_err(c, LBM_ERR_HIP, std::string("lbm_sync: ") + hipGetErrorString(e));
+        if (bad)
+            return set_err(c, LBM_ERR_UNSTABLE, std::to_string(bad) + " nodes hold a non-finite rest population at step " +
+                                                    std::to_string(c->step));
+    }
+    return LBM_OK;
+}
+
+extern "C" int lbm_set_step(lbm_ctx *c, long step) {
+    if (!c) return LBM_ERR_INVALID;
+    if (!c->initialized) return set_err(c, LBM_ERR_STATE, "lbm_set_step before lbm_initialize");
+    if (c->stage != 0) return set_err(c, LBM_ERR_STATE, "lbm_set_step inside a staged step");
+    if (step < 0) return set_err(c, LBM_ERR_INVALID, "negative step");
+    c->step = step;
     return LBM_OK;
 }
 

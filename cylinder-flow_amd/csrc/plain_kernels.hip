@@ -20,6 +20,8 @@
 //   source x == nx (outlet, x not periodic)        -> zero gradient, source clamped to nx-1
 //   source is solid (mask)                         -> half-way bounce-back, momentum exchange
 //   otherwise                                      -> g_i = h_i(node - e_i) (periodic wrap)
+#include <type_traits>
+
 #include "fused_common.hpp"
 
 namespace lbm {
@@ -255,6 +257,33 @@ __global__ void __launch_bounds__(512) k_macros_extract(const double *__restrict
     out[(size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * z)] = flags[node] == FL_SOLID ? 0.0 : mac[node];
 }
 
+// Health check of a population buffer (lbm_sync): counts the non-finite values of one direction
+// over a box of nodes, element base + x + sy*y + sz*z (x < nx; y, z the grid), skipping the nodes
+// flagged solid when flags is set (flags at x + fpitch*(y + ny*z)).  A non-finite population
+// anywhere reaches the rest population of the nodes that pull it within one step, through the
+// moments, so one direction of the live nodes suffices.  16-bit formats are tested on their bits
+// (exponent field all ones), FP32/FP64 with isfinite.
+template <class T>
+__device__ __forceinline__ bool finite_value(T v) {
+    if constexpr (sizeof(T) == 8) return __builtin_isfinite((double)v);
+    else if constexpr (sizeof(T) == 4) return __builtin_isfinite((float)v);
+    else {
+        uint16_t b;
+        __builtin_memcpy(&b, &v, 2);
+        return std::is_same<T, _Float16>::value ? (b & 0x7c00u) != 0x7c00u : (b & 0x7f80u) != 0x7f80u;
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_count_nonfinite(const T *__restrict__ base, size_t sy, size_t sz, int nx,
+                                                         const uint8_t *__restrict__ flags, int fpitch, int ny,
+                                                         unsigned *__restrict__ bad) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+    if (x >= nx) return;
+    if (flags && flags[(size_t)x + (size_t)fpitch * ((size_t)y + (size_t)ny * z)] == FL_SOLID) return;
+    if (!finite_value(base[(size_t)x + sy * y + sz * z])) atomicAdd(bad, 1u);
+}
+
 // ------------------------------------------------------------------------------------
 // host launchers
 // ------------------------------------------------------------------------------------
@@ -386,6 +415,19 @@ hipError_t plain_encode(int Q, int prec, void *A, const double *in, const uint8_
         using T = typename S::T;
         hipLaunchKernelGGL(k_encode<S>, grid_of(a.g, sh), dim3(sh.bx), 0, s, (T *)A + (size_t)dir * a.g.ds,
                            in, flags, (float)(1.0 / w), a.sp.offw[dir], a.g);
+        return hipGetLastError();
+    });
+}
+
+hipError_t count_nonfinite(int prec, const void *base, size_t sy, size_t sz, int nx, int ny, int nz,
+                           const uint8_t *flags, int fpitch, unsigned *bad, hipStream_t s) {
+    hipError_t e = hipMemsetAsync(bad, 0, sizeof(unsigned), s);
+    if (e != hipSuccess) return e;
+    const dim3 grid((unsigned)((nx + 255) / 256), (unsigned)ny, (unsigned)nz);
+    return dispatch_store(prec, [&](auto st) {
+        using T = typename decltype(st)::T;
+        hipLaunchKernelGGL(k_count_nonfinite<T>, grid, dim3(256), 0, s, (const T *)base, sy, sz, nx, flags, fpitch, ny,
+                           bad);
         return hipGetLastError();
     });
 }

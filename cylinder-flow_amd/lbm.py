@@ -75,6 +75,7 @@ def lib():
             "lbm_step": (I, [P, I]),
             "lbm_step_timed": (I, [P, I, C.POINTER(C.c_double)]),
             "lbm_sync": (I, [P]),
+            "lbm_set_step": (I, [P, C.c_long]),
             "lbm_collide": (I, [P]),
             "lbm_boundary": (I, [P]),
             "lbm_stream": (I, [P]),
@@ -245,6 +246,9 @@ class Solver:
         a = np.ascontiguousarray(f, np.float64)
         assert a.size == self.Q * self.N
         self._chk(lib().lbm_set_populations(self.h, _p(a)))
+
+    def set_step(self, step):
+        self._chk(lib().lbm_set_step(self.h, int(step)))
 
     def forces(self):
         n = C.c_int()

@@ -34,7 +34,7 @@ extern "C" {
 #define LBM_ERR_HIP 2       /* HIP runtime error (message in lbm_last_error) */
 #define LBM_ERR_NOMEM 3     /* device or host allocation failed */
 #define LBM_ERR_STATE 4     /* call out of order (e.g. step before initialise) */
-#define LBM_ERR_UNSTABLE 5  /* a recorded force is NaN/Inf or exceeds the limit */
+#define LBM_ERR_UNSTABLE 5  /* a recorded force or a population is NaN/Inf, or a force exceeds the limit */
 
 enum lbm_precision { LBM_FP64 = 0, LBM_FP32 = 1, LBM_FP16S = 2, LBM_BF16S = 3 };
 /* LBM_BC_REFERENCE: the reference's own boundary rules and kernel order (ghost layers,
@@ -148,17 +148,26 @@ int lbm_collide(lbm_ctx *ctx);
 int lbm_boundary(lbm_ctx *ctx);
 int lbm_stream(lbm_ctx *ctx);
 int lbm_update(lbm_ctx *ctx);
-/* drains the context stream and checks the forces recorded since the last check */
+/* drains the context stream, checks the forces recorded since the last check (NaN/Inf or above
+ * force_limit) and the state itself (a non-finite rest population at any live node), and reports
+ * either as LBM_ERR_UNSTABLE; the reference has no failure detection */
 int lbm_sync(lbm_ctx *ctx);
 
 /* macroscopic fields after the last step (the arrays Update writes:
- * p, ua, va, wa -- 3D MPI+KOKKOS+MPIIO/lbm.cpp:384-415); any pointer may be NULL */
+ * p, ua, va, wa -- 3D MPI+KOKKOS+MPIIO/lbm.cpp:384-415); any pointer may be NULL.  Steps
+ * write only populations: the first read after a step runs that step again from its input,
+ * with the moments on (same populations, bit for bit; the cost of one step), which is how
+ * the reference's every-step Update becomes an output-step cost (main.cpp:39-44). */
 int lbm_get_fields(lbm_ctx *ctx, double *p, double *u, double *v, double *w);
 /* populations (decoded to FP64), Q*N doubles.  Reference mode: the reference's f after
  * Update (streamed, not yet collided: 3D MPI+KOKKOS+MPIIO/lbm.cpp:359-370).  Plain mode:
  * the stored state (post-collision), which lbm_set_populations replaces (restart). */
 int lbm_get_populations(lbm_ctx *ctx, double *f);
 int lbm_set_populations(lbm_ctx *ctx, const double *f);
+/* the step counter (lbm_info.step), which also fixes the force-record phase (steps with
+ * step % force_interval == 0): with lbm_get/set_populations a checkpoint and restart, which the
+ * reference does not have; bit-identical continuation in FP64 and FP32 storage */
+int lbm_set_step(lbm_ctx *ctx, long step);
 /* momentum-exchange force records (step, Fx, Fy, Fz): MPI Kokkos/lbm.cpp:356-407.
  * Copies min(max, available) records, sets *count to the number available. */
 int lbm_get_forces(lbm_ctx *ctx, double *out, int max, int *count);
