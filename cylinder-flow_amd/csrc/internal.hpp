@@ -192,6 +192,8 @@ struct lbm_ctx {
     unsigned fast_lds = 0;
 
     unsigned *dbad = nullptr;       // lbm_sync's count of non-finite populations (device)
+    double *noalloc_scratch = nullptr;   // lbm_get_fields_noalloc's staging array on this device
+    std::string noalloc_err;
 
     // force history (device + host copy)
     double *hist = nullptr;
@@ -225,3 +227,5 @@ struct lbm_ctx {
 
 // materialise the fields of the last step (see lbm_capi.cpp); LBM_OK or an LBM_ERR_* code
 int ensure_fields(lbm_ctx *c);
+// the address of lbm_get_fields_noalloc's staging array on the current device (noalloc.hip)
+int noalloc_setup(double **out, std::string &err);

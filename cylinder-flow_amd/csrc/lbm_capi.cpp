@@ -221,6 +221,8 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_ctx **out) {
         delete c;
         return LBM_ERR_HIP;
     }
+    // lbm_get_fields_noalloc's staging array on this device (its own error, reported by that call)
+    if (noalloc_setup(&c->noalloc_scratch, c->noalloc_err) != LBM_OK) c->noalloc_scratch = nullptr;
     *out = c;
     return LBM_OK;
 }

@@ -75,6 +75,7 @@ def lib():
             "lbm_step": (I, [P, I]),
             "lbm_step_timed": (I, [P, I, C.POINTER(C.c_double)]),
             "lbm_sync": (I, [P]),
+            "lbm_get_fields_noalloc": (I, [P, P, P, P, P]),
             "lbm_set_step": (I, [P, C.c_long]),
             "lbm_collide": (I, [P]),
             "lbm_boundary": (I, [P]),

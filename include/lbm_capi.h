@@ -159,6 +159,12 @@ int lbm_sync(lbm_ctx *ctx);
  * with the moments on (same populations, bit for bit; the cost of one step), which is how
  * the reference's every-step Update becomes an output-step cost (main.cpp:39-44). */
 int lbm_get_fields(lbm_ctx *ctx, double *p, double *u, double *v, double *w);
+/* lbm_get_fields without allocating: no counterpart in the reference; for callers that cannot
+ * allocate (tight serving or monitoring loops, hosts that forbid allocation after set-up).  The
+ * fields are staged through one fixed device array of 32 MiB per GPU (held until the process
+ * ends), so at most 4 194 304 nodes (LBM_ERR_INVALID above); calls that overlap on one GPU share
+ * that array and race -- serialise them. */
+int lbm_get_fields_noalloc(lbm_ctx *ctx, double *p, double *u, double *v, double *w);
 /* populations (decoded to FP64), Q*N doubles.  Reference mode: the reference's f after
  * Update (streamed, not yet collided: 3D MPI+KOKKOS+MPIIO/lbm.cpp:359-370).  Plain mode:
  * the stored state (post-collision), which lbm_set_populations replaces (restart). */
