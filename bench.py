@@ -135,17 +135,67 @@ def cpu_reference_structured(seconds=10.0):
             "kind": "port", "sample": f"FP64 oracle reference mode, {n} steps of a 512x256x32 slab ({dt:.1f} s)"}
 
 
-def adjoint_ratio(pkg, steps=500):
-    """Adjoint (recompute + reverse sweep) over forward time on the D3Q27 adjoint workload's
-    192x96x96 geometry, FP32, window [steps/2, steps], checkpoint every 250 steps."""
+def adjoint_ratio(pkg):
+    """Adjoint (recompute + reverse sweep) over forward time on the D3Q27 adjoint workload:
+    192x96x96, FP32, J = mean Cd over steps 2000-6000, 6000 steps, checkpoint every 250."""
     W = pkg.configs.adjoint3d_192x96x96()
+    n0, n1 = W.window
     s = pkg.Solver(W.cfg)
     s.set_solid_fraction(W.extra["sfrac"])
     s.initialize()
-    J, ds, du, tf, ta = s.adjoint(steps // 2, steps, W.cd_scale(), 250)
+    J, ds, du, tf, ta = s.adjoint(n0, n1, W.cd_scale(), W.checkpoint)
     s.close()
-    return {"workload": "d3q27_adjoint_192x96x96 (fp32), %d steps, checkpoint 250" % steps,
-            "forward_ms": round(tf, 3), "adjoint_ms": round(ta, 3), "ratio": round(ta / tf, 3)}
+    return {"workload": f"d3q27_adjoint_192x96x96 (fp32), J over steps {n0}-{n1}, checkpoint {W.checkpoint}",
+            "forward_ms": round(tf, 3), "adjoint_ms": round(ta, 3), "ratio": round(ta / tf, 3), "J": J}
+
+
+def config_legs(pkg):
+    """The two other forward configurations at their own sizes.  Config 0 (D2Q9 400x100 cylinder,
+    FP64, drag every 10 steps): time per step over 2000 steps, several steps per launch (the
+    default on small domains) against one launch per step.  Config 1 (D3Q27 256x128x128 sphere):
+    MLUPS over 200 steps in each storage format, and after its 10 000 steps the FP32 and FP16
+    runs' mean drag (second half) and velocity against the FP64 run."""
+    import numpy as np
+    out = {}
+    W = pkg.configs.cylinder2d_400x100()
+    n = W.cfg.nx * W.cfg.ny
+    c0 = {"workload": W.name + " fp64, drag every 10 steps"}
+    for tb, key in (("1", "several_steps_per_launch"), ("0", "one_launch_per_step")):
+        os.environ["LBM_TEMPORAL_BLOCKING"] = tb
+        with pkg.Solver(W.cfg) as s:
+            s.initialize()
+            s.step_timed(20)
+            ms = s.step_timed(2000)
+        c0[key] = {"us_per_step": round(ms / 2000 * 1e3, 3), "mlups": round(n * 2000 / ms / 1e3, 1)}
+    os.environ.pop("LBM_TEMPORAL_BLOCKING", None)
+    c0["speedup"] = round(c0["one_launch_per_step"]["us_per_step"] / c0["several_steps_per_launch"]["us_per_step"], 2)
+    out["config0"] = c0
+    log(f"config 0: {c0}")
+    L = pkg.lbm
+    c1 = {"workload": "d3q27_sphere_256x128x128_re200"}
+    ref = None
+    for name, prec in (("fp64", L.FP64), ("fp32", L.FP32), ("fp16s", L.FP16S)):
+        W = pkg.configs.sphere3d_256x128x128(precision=prec)
+        with pkg.Solver(W.cfg) as s:
+            s.initialize()
+            s.step_timed(10)
+            ms = s.step_timed(200)
+            s.step(W.steps - 210)
+            F = s.forces()
+            _, u, v, w = s.fields()
+        sel = F[:, 0] > W.steps // 2
+        rec = {"mlups": round(W.cfg.nx * W.cfg.ny * W.cfg.nz * 200 / ms / 1e3, 1),
+               "cd_mean": float(W.cd_scale() * F[sel, 1].mean())}
+        vel = np.stack([u, v, w])
+        if ref is None:
+            ref = (rec["cd_mean"], vel)
+        else:
+            rec["cd_rel_err"] = float(abs(rec["cd_mean"] - ref[0]) / abs(ref[0]))
+            rec["vel_l2_rel_err"] = float(np.linalg.norm(vel - ref[1]) / np.linalg.norm(ref[1]))
+        c1[name] = rec
+        log(f"config 1 {name}: {rec}")
+    out["config1"] = c1
+    return out
 
 
 def accuracy(pkg, steps, early, window_start=None):
@@ -270,6 +320,9 @@ def main():
                     help="instead of the benchmark, time the reference's default 3D and 2D configurations")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="instead of the benchmark, time the adjoint leg alone (forward and reverse sweep)")
+    ap.add_argument("--configs-only", action="store_true",
+                    help="instead of the benchmark, run the config 0 / config 1 legs alone")
+    ap.add_argument("--no-configs", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the whole domain runs on a single MI355X; "
@@ -286,6 +339,9 @@ def main():
         return
     if a.adjoint_only:
         print(json.dumps({"adjoint": adjoint_ratio(pkg)}), flush=True)
+        return
+    if a.configs_only:
+        print(json.dumps({"configs": config_legs(pkg)}), flush=True)
         return
 
     log(f"fp16s leg: {K} steps after {Wm} warm-up")
@@ -330,6 +386,9 @@ def main():
     if not a.no_adjoint:
         log("adjoint leg")
         out["adjoint"] = adjoint_ratio(pkg)
+    if not a.no_configs:
+        log("config 0 / config 1 legs")
+        out["configs"] = config_legs(pkg)
     if not a.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)

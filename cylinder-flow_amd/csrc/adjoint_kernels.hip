@@ -21,11 +21,75 @@
 
 namespace lbm {
 
+// Transposes of the inlet's regularisation and Zou-He completion applied to m = mu_m at an inlet
+// node (x = 0, x not periodic), in place, and their contribution to dJ/du_in of its row.
+template <int Q, class S>
+__device__ __forceinline__ void adj_inlet_transpose(typename S::C *m, double &du) {
+    using L = Lat<Q>;
+    using C = typename S::C;
+    // regularisation g_i = w_i (S + 3 e_ix U + 4.5 Q_i:P) is g -> A g + 3 w e_x U with
+    // A_ij = w_i (1 + 4.5 Q_i:Q_j), so (A^T m)_j = M0 + 4.5 Q_j:T, M0 = sum w_i m_i,
+    // T = sum w_i m_i Q_i (same structure as the forward with S -> M0, P -> T)
+    {
+        C M0 = 0, Rxx = 0, Ryy = 0, Rzz = 0, Rxy = 0, Rxz = 0, Ryz = 0;
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i);
+            const C wm = (C)L::w(i) * m[i];
+            M0 += wm;
+            if constexpr (ex != 0) Rxx += wm;
+            if constexpr (ey != 0) Ryy += wm;
+            if constexpr (ez != 0) Rzz += wm;
+            if constexpr (ex * ey != 0) Rxy += (C)(ex * ey) * wm;
+            if constexpr (ex * ez != 0) Rxz += (C)(ex * ez) * wm;
+            if constexpr (ey * ez != 0) Ryz += (C)(ey * ez) * wm;
+            if constexpr (ex != 0) du += (double)wm * (3.0 * ex);
+        });
+        const C M3 = M0 * (C)(1.0 / 3.0);
+        const C Txx = Rxx - M3, Tyy = Ryy - M3, Tzz = Rzz - M3;
+        const C d = (L::D == 3 ? Txx + Tyy + Tzz : Txx + Tyy) * (C)(1.0 / 3.0);
+        static_for<0, Q>([&](auto J) {
+            constexpr int j = J;
+            constexpr int ex = L::ex(j), ey = L::ey(j), ez = L::ez(j);
+            C q = -d, o = 0;
+            if constexpr (ex != 0) q += Txx;
+            if constexpr (ey != 0) q += Tyy;
+            if constexpr (ez != 0) q += Tzz;
+            if constexpr (ex * ey != 0) o += (C)(ex * ey) * Rxy;
+            if constexpr (ex * ez != 0) o += (C)(ex * ez) * Rxz;
+            if constexpr (ey * ez != 0) o += (C)(ey * ez) * Ryz;
+            m[j] = M0 + ((C)4.5 * q + (C)9 * o);
+        });
+    }
+    // Zou-He g_i = g_opp(i) + 6 w_i U - 18 w_i (e_iy N_y + e_iz N_z), N_t = sum_{e_kx=0} g_k e_kt
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = I;
+        if constexpr (L::ex(i) > 0) {
+            const C mi = m[i];
+            m[L::opp(i)] += mi;
+            static_for<0, Q>([&](auto K) {
+                constexpr int k = K;
+                constexpr int dot = L::ey(i) * L::ey(k) + L::ez(i) * L::ez(k);
+                if constexpr (L::ex(k) == 0 && dot != 0) m[k] += mi * (C)(-18.0 * L::w(i) * dot);
+            });
+            du += (double)mi * (6.0 * L::w(i));
+            m[i] = 0;
+        }
+    });
+}
+
+// Interior rows (no wall or periodic wrap in y/z) whose nodes are all fluid and have no solid
+// neighbour ("clean", flagged per row on the host) take a straight-line path: every source and
+// destination row of a direction is a scalar of the block, and the x edges (inlet, outlet,
+// periodic x) are handled with in-row clamped indices and selects, so every load of the path is
+// issued unconditionally from an address inside its row.  The arithmetic is the general path's,
+// term for term, so both give the same bits.  Other rows take the general path.
 template <int Q, class S, bool SFRAC>
 __global__ void __launch_bounds__(256)
     k_adj_step(const typename S::T *__restrict__ H, const typename S::C *__restrict__ mun,
                typename S::C *__restrict__ mu, const uint8_t *__restrict__ flags, const typename S::C *__restrict__ sfrac,
-               const double *__restrict__ uin, double *__restrict__ dJds, double *__restrict__ dJdu, const AdjArgs a) {
+               const double *__restrict__ uin, double *__restrict__ dJds, double *__restrict__ dJdu,
+               const uint8_t *__restrict__ rowclean, const AdjArgs a) {
     using L = Lat<Q>;
     using C = typename S::C;
     const Geo &g = a.f.g;
@@ -33,14 +97,71 @@ __global__ void __launch_bounds__(256)
     if (x >= g.nx) return;
     const uint32_t row = (uint32_t)y + (uint32_t)g.ny * (uint32_t)z;
     const uint32_t node = (uint32_t)x + (uint32_t)g.nxp * row;
-    const uint8_t fl = flags[node];
+    const bool interior = (unsigned)(y - 1) < (unsigned)(g.ny - 2) && (L::D == 2 || (unsigned)(z - 1) < (unsigned)(g.nz - 2));
+    const bool fast = interior && rowclean[row];   // uniform over the block (a block lies in one row)
+    const uint8_t fl = fast ? (uint8_t)FL_FLUID : flags[node];
     if (fl == FL_SOLID) {
         static_for<0, Q>([&](auto I) { mu[pel(g, I, (uint32_t)x, row)] = (C)0; });
         return;
     }
     const C an = (C)a.alpha_next;
-    // ---- 1. lambda = dJ/dH_m(node) ----
+    const bool xlo = !g.xper && x == 0, xhi = !g.xper && x == g.nx - 1;
     C lam[Q];
+    C gv[Q];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (fast) {
+        // ---- 1. lambda: entry (node, opp i) pulls from node + e_i, entry (d, i) from node;
+        //         the outlet node also feeds (nx-1, y + e_y, z + e_z) along e_ix < 0 ----
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i);
+            const C *mrow = mun + (size_t)i * g.ds + (size_t)(row + (uint32_t)(ey + g.ny * ez)) * g.rs;
+            int dx = x + ex;
+            bool live = true;
+            if constexpr (ex > 0) {
+                if (dx == g.nx) {
+                    if (g.xper) dx = 0;
+                    else live = false, dx = x;   // beyond the outlet: no entry
+                }
+            }
+            if constexpr (ex < 0) {
+                if (dx < 0) {
+                    if (g.xper) dx = g.nx - 1;
+                    else live = false, dx = x;   // (node, opp i) is a Zou-He unknown
+                }
+            }
+            const C v = mrow[dx];
+            lam[i] = live ? v : (C)0;
+        });
+        if (xhi) {   // the outlet lane only
+            static_for<0, Q>([&](auto I) {
+                constexpr int i = I;
+                constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i);
+                if constexpr (ex < 0)
+                    lam[i] += mun[(size_t)i * g.ds + (size_t)(row + (uint32_t)(ey + g.ny * ez)) * g.rs + (size_t)x];
+            });
+        }
+        // ---- 2. recompute G_m(node): the forward pull (inlet unknowns 0, outlet clamp) ----
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i);
+            const typename S::T *hrow = H + (size_t)i * g.ds + (size_t)(row - (uint32_t)(ey + g.ny * ez)) * g.rs;
+            int sx = x - ex;
+            bool zh = false;
+            if constexpr (ex > 0) {
+                if (sx < 0) {
+                    if (g.xper) sx = g.nx - 1;
+                    else zh = true, sx = 0;
+                }
+            }
+            if constexpr (ex < 0) {
+                if (sx == g.nx) sx = g.xper ? 0 : g.nx - 1;
+            }
+            const C v = S::template decode<i, L>(hrow[sx], a.f.sp.off[i]);
+            gv[i] = zh ? (C)0 : v;
+        });
+    } else {
+    // ---- 1. lambda = dJ/dH_m(node) ----
     static_for<0, Q>([&](auto I) {
         constexpr int i = I;
         constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i), o = L::opp(i);
@@ -102,10 +223,9 @@ __global__ void __launch_bounds__(256)
     });
 
     // ---- 2. recompute G_m(node) ----
-    C gv[Q];
-    double fx = 0.0, fy = 0.0, fz = 0.0;
     pull<Q, S, false>(H, flags, g, a.f.sp, x, y, z, node, true, fl == FL_NEAR, gv, fx, fy, fz);
-    const bool inlet = !g.xper && x == 0;
+    }
+    const bool inlet = xlo;
     if (inlet) velocity_inlet<Q, C>(gv, (C)uin[y + g.ny * z]);
 
     // ---- 3. local transpose ----
@@ -154,60 +274,14 @@ __global__ void __launch_bounds__(256)
                al * (C)2 * sf * (C)L::ex(j);
     });
     if (inlet) {
-        // regularisation g_i = w_i (S + 3 e_ix U + 4.5 Q_i:P) is g -> A g + 3 w e_x U with
-        // A_ij = w_i (1 + 4.5 Q_i:Q_j), so (A^T m)_j = M0 + 4.5 Q_j:T, M0 = sum w_i m_i,
-        // T = sum w_i m_i Q_i (same structure as the forward with S -> M0, P -> T)
         double du = 0.0;
-        {
-            C M0 = 0, Rxx = 0, Ryy = 0, Rzz = 0, Rxy = 0, Rxz = 0, Ryz = 0;
-            static_for<0, Q>([&](auto I) {
-                constexpr int i = I;
-                constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i);
-                const C wm = (C)L::w(i) * m[i];
-                M0 += wm;
-                if constexpr (ex != 0) Rxx += wm;
-                if constexpr (ey != 0) Ryy += wm;
-                if constexpr (ez != 0) Rzz += wm;
-                if constexpr (ex * ey != 0) Rxy += (C)(ex * ey) * wm;
-                if constexpr (ex * ez != 0) Rxz += (C)(ex * ez) * wm;
-                if constexpr (ey * ez != 0) Ryz += (C)(ey * ez) * wm;
-                if constexpr (ex != 0) du += (double)wm * (3.0 * ex);
-            });
-            const C M3 = M0 * (C)(1.0 / 3.0);
-            const C Txx = Rxx - M3, Tyy = Ryy - M3, Tzz = Rzz - M3;
-            const C d = (L::D == 3 ? Txx + Tyy + Tzz : Txx + Tyy) * (C)(1.0 / 3.0);
-            static_for<0, Q>([&](auto J) {
-                constexpr int j = J;
-                constexpr int ex = L::ex(j), ey = L::ey(j), ez = L::ez(j);
-                C q = -d, o = 0;
-                if constexpr (ex != 0) q += Txx;
-                if constexpr (ey != 0) q += Tyy;
-                if constexpr (ez != 0) q += Tzz;
-                if constexpr (ex * ey != 0) o += (C)(ex * ey) * Rxy;
-                if constexpr (ex * ez != 0) o += (C)(ex * ez) * Rxz;
-                if constexpr (ey * ez != 0) o += (C)(ey * ez) * Ryz;
-                m[j] = M0 + ((C)4.5 * q + (C)9 * o);
-            });
-        }
-        // Zou-He g_i = g_opp(i) + 6 w_i U - 18 w_i (e_iy N_y + e_iz N_z), N_t = sum_{e_kx=0} g_k e_kt
-        static_for<0, Q>([&](auto I) {
-            constexpr int i = I;
-            if constexpr (L::ex(i) > 0) {
-                const C mi = m[i];
-                m[L::opp(i)] += mi;
-                static_for<0, Q>([&](auto K) {
-                    constexpr int k = K;
-                    constexpr int dot = L::ey(i) * L::ey(k) + L::ez(i) * L::ez(k);
-                    if constexpr (L::ex(k) == 0 && dot != 0) m[k] += mi * (C)(-18.0 * L::w(i) * dot);
-                });
-                du += (double)mi * (6.0 * L::w(i));
-                m[i] = 0;
-            }
-        });
+        adj_inlet_transpose<Q, S>(m, du);
         dJdu[y + (size_t)g.ny * z] += du;
     }
-    static_for<0, Q>([&](auto I) { mu[pel(g, I, (uint32_t)x, row)] = m[I]; });
+    static_for<0, Q>([&](auto I) { __builtin_nontemporal_store(m[I], &mu[pel(g, I, (uint32_t)x, row)]); });
 }
+
+
 
 // initial condition H_0 = feq(0, (u_in(y,z) + perturb*xi, 0, 0)): dJ/du_in(row) +=
 // sum over the row's fluid nodes of sum_i lambda_0,i dfeq_i/du_x, lambda_0 = transpose of
@@ -303,27 +377,31 @@ __global__ void __launch_bounds__(256) k_adj_strip(const double *__restrict__ in
 // ------------------------------------------------------------------------------------
 template <int Q, class S>
 static hipError_t adj_step_qs(const void *H, const void *mun, void *mu, const uint8_t *flags, const void *sfrac,
-                              const double *uin, void *dJds, double *dJdu, const AdjArgs &a, hipStream_t s) {
+                              const double *uin, void *dJds, double *dJdu, const uint8_t *rowclean, const AdjArgs &a,
+                              hipStream_t s) {
     using T = typename S::T;
     using C = typename S::C;
-    const dim3 grid((a.f.g.nx + 255) / 256, a.f.g.ny, a.f.g.nz);
+    // blocks of one row's lanes (at most 256), so the fast path's test is uniform per block
+    const Shape sh = lane_shape(a.f.g, 1);
+    const dim3 grid(sh.gx, a.f.g.ny, a.f.g.nz);
     if (sfrac)
-        hipLaunchKernelGGL((k_adj_step<Q, S, true>), grid, dim3(256), 0, s, (const T *)H, (const C *)mun, (C *)mu, flags,
-                           (const C *)sfrac, uin, (double *)dJds, dJdu, a);
+        hipLaunchKernelGGL((k_adj_step<Q, S, true>), grid, dim3(sh.bx), 0, s, (const T *)H, (const C *)mun, (C *)mu, flags,
+                           (const C *)sfrac, uin, (double *)dJds, dJdu, rowclean, a);
     else
-        hipLaunchKernelGGL((k_adj_step<Q, S, false>), grid, dim3(256), 0, s, (const T *)H, (const C *)mun, (C *)mu, flags,
-                           (const C *)nullptr, uin, (double *)dJds, dJdu, a);
+        hipLaunchKernelGGL((k_adj_step<Q, S, false>), grid, dim3(sh.bx), 0, s, (const T *)H, (const C *)mun, (C *)mu, flags,
+                           (const C *)nullptr, uin, (double *)dJds, dJdu, rowclean, a);
     return hipGetLastError();
 }
 
 hipError_t adj_step(int Q, int prec, const void *H, const void *mun, void *mu, const uint8_t *flags, const void *sfrac,
-                    const double *uin, void *dJds, double *dJdu, const AdjArgs &a, hipStream_t s) {
+                    const double *uin, void *dJds, double *dJdu, const uint8_t *rowclean, const AdjArgs &a,
+                    hipStream_t s) {
     if (prec == LBM_FP64)
-        return Q == 9 ? adj_step_qs<9, SF64>(H, mun, mu, flags, sfrac, uin, dJds, dJdu, a, s)
-                      : adj_step_qs<27, SF64>(H, mun, mu, flags, sfrac, uin, dJds, dJdu, a, s);
+        return Q == 9 ? adj_step_qs<9, SF64>(H, mun, mu, flags, sfrac, uin, dJds, dJdu, rowclean, a, s)
+                      : adj_step_qs<27, SF64>(H, mun, mu, flags, sfrac, uin, dJds, dJdu, rowclean, a, s);
     if (prec == LBM_FP32)
-        return Q == 9 ? adj_step_qs<9, SF32>(H, mun, mu, flags, sfrac, uin, dJds, dJdu, a, s)
-                      : adj_step_qs<27, SF32>(H, mun, mu, flags, sfrac, uin, dJds, dJdu, a, s);
+        return Q == 9 ? adj_step_qs<9, SF32>(H, mun, mu, flags, sfrac, uin, dJds, dJdu, rowclean, a, s)
+                      : adj_step_qs<27, SF32>(H, mun, mu, flags, sfrac, uin, dJds, dJdu, rowclean, a, s);
     return hipErrorInvalidValue;
 }
 

@@ -17,6 +17,19 @@ __device__ __forceinline__ double uniform_pm1(uint64_t seed, uint64_t index) {
     return (double)(splitmix64(seed ^ splitmix64(index)) >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
 }
 
+// a / b rounded to nearest, given y = 1/b rounded to nearest: q = RN(a y), r = a - b q (exact
+// with one FMA), RN(q + r y).  Markstein's correction (P. W. Markstein, IBM J. Res. Dev. 34(1),
+// 1990; the final step of IA-64 division): with a correctly rounded reciprocal and q within an
+// ulp of a/b it yields the correctly rounded quotient for quotients in the normal range, the same
+// bits as IEEE division in three FMA-unit operations instead of the general sequence.  Used for
+// the FP64 path's divisions by 3 and by tau (checked against IEEE division on the CPU:
+// tests/div_check.c).
+__device__ __forceinline__ double div_cr(double a, double b, double y) {
+    const double q = a * y;
+    const double r = __builtin_fma(-q, b, a);
+    return __builtin_fma(r, y, q);
+}
+
 // shifted incompressible equilibrium in the reference's rounding order
 // (3D MPI+KOKKOS+MPIIO/lbm.cpp:263-267):  t*p*3 + t*(3 edu + 4.5 edu^2 - 1.5 udu)
 template <int Q, int I>

@@ -36,6 +36,7 @@ __host__ __device__ __forceinline__ size_t pel(const Geo &g, int i, uint32_t x, 
 struct FusedArgs {
     Geo g;
     double tau;      // tau0 + 0.5 (FP64 path: f - (f - feq)/tau, the reference's form)
+    double rtau;     // 1/tau rounded to nearest (for div_cr)
     float omega;     // 1/(tau0 + 0.5) (FP32 path)
     StoreParams sp;
 };
@@ -53,6 +54,10 @@ hipError_t plain_init(int Q, int prec, void *A, void *B, double *macros, const u
                       const double *xi, double perturb, uint64_t seed, const FusedArgs &a, hipStream_t s);
 // launch shape of the fused kernels: V nodes per lane, at most 256 lanes per block
 Shape lane_shape(const Geo &g, int V);
+// launch shape of the FP64 kernel k_fused (and of its per-block force slots): 64-lane blocks on
+// domains whose population buffer is at most 16 MB (the step is latency-bound there and wants
+// every CU), otherwise lane_shape(g, 1)
+Shape fp64_shape(const Geo &g, int Q);
 
 // buffers of one fused step
 struct StepBufs {
@@ -87,6 +92,15 @@ hipError_t plain_step(int Q, int prec, int V, const StepBufs &b, bool force, boo
 hipError_t fast_step_f32(int Q, int V, const StepBufs &b, bool force, bool mac, const FusedArgs &a, hipStream_t s);
 hipError_t fast_step_f16(int Q, int V, const StepBufs &b, bool force, bool mac, const FusedArgs &a, hipStream_t s);
 hipError_t fast_step_bf16(int Q, int V, const StepBufs &b, bool force, bool mac, const FusedArgs &a, hipStream_t s);
+// several D2Q9 FP64 steps per launch (small_kernels.hip): tb_depth() steps at most, T x T tiles
+// (tb_tile), tb_tiles(g, T, K) blocks with tb_lds_bytes(T) of LDS each; tb_prepare() before use
+int tb_depth();
+size_t tb_lds_bytes(int T);
+int tb_tiles(const Geo &g, int T, int K, int *tiles_x);
+int tb_tile(const Geo &g, int K, int cus);
+hipError_t tb_prepare();
+hipError_t tb_steps(const void *A, void *B, const uint8_t *flags, const double *uin, double *partials, double *hist,
+                    int rec0, int *nrec, const FusedArgs &a, int T, int K, int steps, long step0, int fi, hipStream_t s);
 // sum of the nslots force partials into hist[rec] (chunks: force_chunks(nslots) * 3 doubles)
 int force_chunks(int nslots);
 hipError_t force_reduce(const double *partials, int nslots, double *chunks, double *hist, int rec, long step,
@@ -139,9 +153,10 @@ struct AdjArgs {
     double alpha;        // window weight of this step (dJ/dFx)
     double alpha_next;   // window weight of the next step
 };
+// rowclean: per row (ny*nz bytes), 1 when no node of the row is solid or next to a solid node
 hipError_t adj_step(int Q, int prec, const void *H, const void *mu_next, void *mu, const uint8_t *flags,
-                    const void *sfrac, const double *uin, void *dJds, double *dJdu_rows, const AdjArgs &a,
-                    hipStream_t s);
+                    const void *sfrac, const double *uin, void *dJds, double *dJdu_rows, const uint8_t *rowclean,
+                    const AdjArgs &a, hipStream_t s);
 hipError_t adj_init(int Q, int prec, const void *mu1, const uint8_t *flags, const double *uin,
                     const double *xi, double perturb, uint64_t seed, double alpha1, double *dJdu_rows,
                     const FusedArgs &a, hipStream_t s);
@@ -171,12 +186,16 @@ struct lbm_ctx {
     // plain-mode device state
     void *A = nullptr, *B = nullptr;
     uint8_t *flags = nullptr;
+    uint8_t *rowclean = nullptr;    // per row: 1 when every node is fluid with no solid neighbour
     void *dsfrac = nullptr;
     double *duin = nullptr;
     double *macros = nullptr;       // 4 planes of FP64
     int *block_slot = nullptr;
     double *partials = nullptr;
     int nslots = 0;
+    int tb_k = 0;                   // D2Q9 FP64 small domains: steps per temporally blocked launch (0: off)
+    int tb_t = 0;                   // and its tile side
+    double *tb_partials = nullptr;  // their per-tile force partials
     uint32_t *fix = nullptr;        // FP32/16-bit: nodes recomputed per node after the packed path
     uint32_t *fixlk = nullptr;      // their link words
     int nfix = 0;

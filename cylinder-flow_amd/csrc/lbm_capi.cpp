@@ -92,8 +92,8 @@ extern "C" const char *lbm_last_error(const lbm_ctx *ctx) { return ctx ? ctx->er
 
 static void free_all(lbm_ctx *c) {
     void *ptrs[] = {c->A, c->B, c->flags, c->dsfrac, c->duin, c->macros, c->block_slot, c->partials, c->hist, c->fix,
-                    c->fixlk, c->rowwin, c->wlk, c->link, c->linklk, c->link_slot, c->rf, c->rft, c->rmac, c->rprof, c->rusr, c->rfpre, c->rsurf,
-                    c->snap_dev, c->dbad};
+                    c->fixlk, c->rowwin, c->wlk, c->link, c->linklk, c->link_slot, c->rowclean, c->rf, c->rft, c->rmac, c->rprof, c->rusr, c->rfpre, c->rsurf,
+                    c->snap_dev, c->dbad, c->tb_partials};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c->A = c->B = nullptr;
@@ -106,9 +106,12 @@ static void free_all(lbm_ctx *c) {
     c->wlk = nullptr;
     c->link = c->linklk = nullptr;
     c->link_slot = nullptr;
+    c->rowclean = nullptr;
     c->nfix = c->nlink = 0;
     c->snap_dev = nullptr;
     c->dbad = nullptr;
+    c->tb_partials = nullptr;
+    c->tb_k = 0;
     c->rf = c->rft = c->rmac = c->rprof = c->rfpre = nullptr;
     c->rsurf = nullptr;
     c->nsurf = 0;
@@ -382,6 +385,7 @@ FusedArgs ctx_fused_args(const lbm_ctx *c) {
     std::memset(&a, 0, sizeof(a));
     a.g = c->geo;
     a.tau = c->cfg.tau0 + 0.5;
+    a.rtau = 1.0 / a.tau;
     a.omega = (float)(1.0 / (c->cfg.tau0 + 0.5));
     // 16-bit formats: offset o_i = feq_i(0, (u_ref, 0, 0)) and o_i / w_i
     std::vector<int> e(3 * c->Q), opp(c->Q);
@@ -450,7 +454,7 @@ static int init_plain(lbm_ctx *c) {
     std::vector<int> slot;
     c->nslots = 0;
     auto block_slots = [&]() {
-        const Shape s1 = lane_shape(g, 1);
+        const Shape s1 = c->prec == LBM_FP64 ? fp64_shape(g, Q) : lane_shape(g, 1);
         slot.assign((size_t)s1.gx * g.ny * g.nz, -1);
         c->nslots = 0;
         for (int z = 0; z < g.nz; z++)
@@ -584,6 +588,29 @@ static int init_plain(lbm_ctx *c) {
             HIP_TRY(c, hipMemcpy(c->link_slot, lslot.data(), sizeof(int) * lslot.size(), hipMemcpyHostToDevice));
         }
     }
+    // D2Q9 FP64 on a small domain (population buffer <= 16 MB, where one step costs about a
+    // dependent launch): several steps per launch (temporal blocking, small_kernels.hip), for
+    // walls, inlet, outlet and a mask; LBM_TEMPORAL_BLOCKING=0 / 1 forbids / allows it
+    // regardless of the size.
+    {
+        const char *tbe = std::getenv("LBM_TEMPORAL_BLOCKING");
+        const bool eligible = Q == 9 && c->prec == LBM_FP64 && !g.xper && !g.yper && !sf;
+        const bool small = (size_t)8 * Q * g.plane <= ((size_t)16 << 20);
+        int dev = 0, lds_max = 0, cus = 0;
+        if (eligible && (tbe ? tbe[0] == '1' : small) && hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            tb_prepare() == hipSuccess) {
+            const int K = tb_depth(), T = tb_tile(g, K, cus);
+            if ((size_t)lds_max >= tb_lds_bytes(T) + 512) {
+                c->tb_k = K;
+                c->tb_t = T;
+                int tiles_x = 0;
+                const int nt = tb_tiles(g, T, K, &tiles_x);
+                HIP_TRY(c, hipMalloc(&c->tb_partials, sizeof(double) * 3 * (size_t)nt * K));
+            }
+        }
+    }
     const size_t esz = c->prec == LBM_FP64 ? 8 : c->prec == LBM_FP32 ? 4 : 2;
     HIP_TRY(c, hipMalloc(&c->A, esz * Q * g.plane));
     HIP_TRY(c, hipMalloc(&c->B, esz * Q * g.plane));
@@ -593,6 +620,13 @@ static int init_plain(lbm_ctx *c) {
     HIP_TRY(c, hipMalloc(&c->block_slot, sizeof(int) * std::max<size_t>(slot.size(), 1)));
     HIP_TRY(c, hipMalloc(&c->partials, sizeof(double) * 3 * (std::max(c->nslots, 1) + force_chunks(c->nslots))));
     HIP_TRY(c, hipMemcpyAsync(c->flags, flags.data(), g.plane, hipMemcpyHostToDevice, c->stream));
+    {   // rows whose nodes are all fluid with no solid neighbour (the adjoint's straight-line rows)
+        std::vector<uint8_t> clean((size_t)g.ny * g.nz, 1);
+        for (size_t r = 0; r < clean.size(); r++)
+            for (int x = 0; x < g.nx && clean[r]; x++) clean[r] = flags[(size_t)x + (size_t)g.nxp * r] == FL_FLUID;
+        HIP_TRY(c, hipMalloc(&c->rowclean, clean.size()));
+        HIP_TRY(c, hipMemcpy(c->rowclean, clean.data(), clean.size(), hipMemcpyHostToDevice));
+    }
     HIP_TRY(c, hipMemcpyAsync(c->duin, c->uin.data(), sizeof(double) * c->uin.size(), hipMemcpyHostToDevice, c->stream));
     if (!slot.empty())
         HIP_TRY(c, hipMemcpy(c->block_slot, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice));
@@ -767,7 +801,22 @@ static int step_n(lbm_ctx *c, int n, hipEvent_t *evs = nullptr) {
         return LBM_OK;
     }
     const FusedArgs a = ctx_fused_args(c);
-    for (int k = 0; k < n; k++) {
+    // temporally blocked launches for all but the last step (which stays a one-step launch, so
+    // its input buffer can serve the fields on request); not under per-kernel timing
+    int left = n;
+    while (c->tb_k > 0 && !evs && left > 1) {
+        const int kk = std::min(c->tb_k, left - 1);
+        int nr = 0;
+        HIP_TRY(c, tb_steps(c->A, c->B, c->flags, c->duin, c->tb_partials, c->hist, c->nrec, &nr, a, c->tb_t, c->tb_k, kk,
+                            c->step, fi, c->stream));
+        c->nrec += nr;
+        c->step += kk;
+        left -= kk;
+        std::swap(c->A, c->B);
+        c->fields_fresh = false;
+        c->replay_ok = false;
+    }
+    for (int k = n - left; k < n; k++) {
         c->step++;
         const bool force = fi > 0 && c->step % fi == 0;
         StepBufs b = step_bufs(c, c->A, c->B);

@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(512)
         double p = 0.0, u = 0.0, v = 0.0, w = 0.0;
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
-            p = p + gv[i] / 3.0;
+            p = p + div_cr(gv[i], 3.0, 1.0 / 3.0);
             if constexpr (L::ex(i) > 0) u = u + gv[i];
             else if constexpr (L::ex(i) < 0) u = u - gv[i];
             if constexpr (L::ey(i) > 0) v = v + gv[i];
@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(512)
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
             const double feq = feq_ref<Q, i>(p, u, v, w);
-            double c = gv[i] - (gv[i] - feq) / a.tau;
+            double c = gv[i] - div_cr(gv[i] - feq, a.tau, a.rtau);
             if constexpr (SFRAC) c = (1.0 - sf) * c + sf * gv[L::opp(i)];
             if (live) S::template store<i, L>(B + (size_t)i * g.ds + pnode, c, a.sp);
         });
@@ -319,6 +319,16 @@ hipError_t plain_init(int Q, int prec, void *A, void *B, double *macros, const u
                   : init_q<27>(prec, A, B, macros, flags, uin, xi, perturb, seed, a, s);
 }
 
+Shape fp64_shape(const Geo &g, int Q) {
+    if ((size_t)8 * Q * g.plane <= ((size_t)16 << 20)) {
+        Shape s;
+        s.bx = 64;
+        s.gx = g.nxp / 64;
+        return s;
+    }
+    return lane_shape(g, 1);
+}
+
 Shape lane_shape(const Geo &g, int V) {
     Shape s;
     const int lanes = g.nxp / V;
@@ -331,7 +341,7 @@ Shape lane_shape(const Geo &g, int V) {
 template <int Q, bool FORCE, bool MACROS>
 static hipError_t step_f64(const StepBufs &b, const FusedArgs &a, hipStream_t s) {
     using S = SF64;
-    const Shape sh = lane_shape(a.g, 1);
+    const Shape sh = fp64_shape(a.g, Q);
     if (b.ev_bulk[0]) {
         const hipError_t e = hipEventRecord(b.ev_bulk[0], s);
         if (e != hipSuccess) return e;

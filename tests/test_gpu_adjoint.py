@@ -148,9 +148,27 @@ def test_adjoint3d_coefficients_and_cost(pkg):
         fd[k] = (vals[0] - vals[1]) / (2 * eps)
     err64 = np.abs(da64 - fd) / np.abs(fd).max()
     err32 = np.abs(da32 - fd) / np.abs(fd).max()
-    print(f"[adjoint] 3D sphere dJ/da FD {fd}\n  fp64 adjoint err {err64.max():.2e}, fp32 adjoint err {err32.max():.2e}, "
-          f"fp32 vs fp64 dJ/ds rel-L2 {rel_l2(ds32, ds64):.2e}")
-    assert err64.max() <= 1e-4
+    # FP64 central differences of dJ/ds at 8 seeded nodes of the smoothed edge (s +- eps inside [0, 1])
+    c.coef[:] = tuple(coef)
+    rng = np.random.default_rng(W.cfg.seed)
+    edge = np.argwhere((sf > 0.05) & (sf < 0.95))
+    pick = [tuple(edge[j]) for j in rng.choice(len(edge), 8, replace=False)]
+    fds = []
+    for i in pick:
+        sp, sm = sf.copy(), sf.copy()
+        sp[i] += eps
+        sm[i] -= eps
+        fds.append((objective(pkg, c, n0, n1, W.cd_scale(), sfrac=sp) -
+                    objective(pkg, c, n0, n1, W.cd_scale(), sfrac=sm)) / (2 * eps))
+    fds = np.array(fds)
+    ref = np.abs(fds).max()
+    es64 = np.abs(np.array([ds64[i] for i in pick]) - fds) / ref
+    es32 = np.abs(np.array([ds32[i] for i in pick]) - fds) / ref
+    print(f"[adjoint] config4 dJ/da FD {fd}\n"
+          f"[adjoint] config4 dJ/da vs FD (max rel to max|FD|): fp64 {err64.max():.2e}, fp32 {err32.max():.2e}\n"
+          f"[adjoint] config4 dJ/ds vs FD at 8 nodes: fp64 {es64.max():.2e}, fp32 {es32.max():.2e}; "
+          f"fp32 vs fp64 dJ/ds rel-L2 {rel_l2(ds32, ds64):.2e}", flush=True)
+    assert err64.max() <= 1e-4 and es64.max() <= 1e-4
     assert err32.max() <= 5e-2
     assert rel_l2(ds32, ds64) <= 5e-2
 

@@ -488,3 +488,50 @@ def test_fields_on_request_leave_the_state_unchanged(pkg, prec, bc, Q):
     for x, y in zip(mid, ref7):
         assert np.array_equal(x, y)
     assert not np.array_equal(mid[1], ma[1])
+
+
+@pytest.mark.parametrize("fi", [0, 1, 3, 10])
+@pytest.mark.parametrize("case", ["disk", "edges"])
+def test_temporal_blocking_equals_one_launch_per_step(pkg, monkeypatch, fi, case):
+    """Several D2Q9 FP64 steps per launch (small domains) against one launch per step: the
+    populations and fields bit for bit, the force records to rounding (summed per tile instead
+    of per row block), for step counts that leave every remainder of the 5-step launches, on a
+    disk and on a mask that touches the inlet and outlet columns and the walls."""
+    L = pkg.lbm
+    nx, ny = 100, 40
+    kw = dict(tau0=0.06, u0=0.05, precision=L.FP64, bc_mode=L.BC_PLAIN, perturb=1e-3, force_interval=fi)
+    mask = None
+    if case == "disk":
+        g = L.make_config(9, nx, ny, 1, body=L.BODY_SPHERE, center=(25.0, 19.5, 0.0), r2=25.0, **kw)
+    else:
+        g = L.make_config(9, nx, ny, 1, body=L.BODY_MASK, **kw)
+        mask = np.zeros((1, ny, nx), np.uint8)
+        mask[0, 2:5, 0] = 1
+        mask[0, 20, 1] = 1
+        mask[0, 3:6, nx - 1] = 1
+        mask[0, ny - 4, nx - 2] = 1
+        mask[0, 0, 40:43] = 1
+        mask[0, 20, 75] = 1
+        mask[0, 21, 50:54] = 1
+    out = []
+    for tb in ("0", "1"):
+        monkeypatch.setenv("LBM_TEMPORAL_BLOCKING", tb)
+        print(f"[parity] temporal blocking {tb} case={case} fi={fi}", flush=True)
+        s = pkg.Solver(g)
+        if mask is not None:
+            s.set_mask(mask)
+        s.initialize()
+        res = []
+        for n in (1, 2, 7, 23, 11):
+            s.step(n)
+            res.append((s.populations(), s.fields()))
+        out.append((res, s.forces()))
+        s.close()
+    (ra, Fa), (rb, Fb) = out
+    for (fa, ma), (fb, mb) in zip(ra, rb):
+        assert np.array_equal(fa, fb)
+        for x, y in zip(ma, mb):
+            assert np.array_equal(x, y)
+    assert Fa.shape == Fb.shape and np.array_equal(Fa[:, 0], Fb[:, 0])
+    if fi:
+        assert np.abs(Fa[:, 1:] - Fb[:, 1:]).max() <= 1e-13 * np.abs(Fa[:, 1:]).max()
