@@ -48,15 +48,34 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+_hip = None
+
+
+def device_sync():
+    """hipDeviceSynchronize: drains every stream, the solver's included (the timing bracket; the
+    solver's own lbm_sync also checks the state for NaN/Inf, which is not part of a step)."""
+    global _hip
+    if _hip is None:
+        import ctypes as C
+        try:
+            _hip = C.CDLL("libamdhip64.so")
+        except OSError:
+            _hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    if _hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipDeviceSynchronize failed")
+
+
 def run_leg(pkg, precision, steps, warmup, force_interval):
     W = pkg.configs.zcylinder3d_512x256x256(precision=precision, force_interval=force_interval)
     s = pkg.Solver(W.cfg).initialize()
     s.step(warmup)
     s.sync()
+    device_sync()
     t0 = time.perf_counter()
     s.step(steps)
-    s.sync()
+    device_sync()
     wall = time.perf_counter() - t0
+    s.sync()                      # forces and state checked for NaN/Inf, after the timed steps
     info = s.info()
     return s, W, wall, info
 
@@ -87,27 +106,49 @@ def traffic_from_profiles(name):
         return None
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(seconds=15.0):
     """The FP64 CPU oracle (push-form port of the plain-mode step, OpenMP) on a slab of the same
-    workload: 512x256x32 nodes with the same cylinder, walls, inlet and outlet."""
+    workload: 512x256x32 nodes with the same cylinder, walls, inlet and outlet, on all the
+    threads OpenMP is given and on one thread (a shorter sample)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     nx, ny, nz = 512, 256, 32
     u, D, Re = 0.05, 40.0, 300.0
     cfg = O.make_cfg(27, nx, ny, nz, tau0=3 * u * D / Re, u0=u, body=O.BODY_ZCYL, center=(128.0, 128.0, 0.0),
                      r2=(D / 2) ** 2, perturb=1e-3, force_interval=10)
-    st = O.PlainState(cfg)
-    st.step(1)
-    t0 = time.perf_counter()
-    st.step(1)
-    one = time.perf_counter() - t0
-    n = max(2, int(seconds / max(one, 1e-3)))
-    t0 = time.perf_counter()
-    st.step(n)
-    dt = time.perf_counter() - t0
-    mlups = nx * ny * nz * n / dt / 1e6
-    return {"value": round(mlups, 2), "unit": "MLUPS", "cores": O.lib().orc_num_threads(), "kind": "port",
-            "sample": f"FP64 oracle, {n} steps of a 512x256x32 slab of the workload ({dt:.1f} s)"}
+    threads = O.lib().orc_num_threads()
+
+    def sample(secs):
+        st = O.PlainState(cfg)
+        st.step(1)
+        t0 = time.perf_counter()
+        st.step(1)
+        one = time.perf_counter() - t0
+        n = max(2, int(secs / max(one, 1e-3)))
+        t0 = time.perf_counter()
+        st.step(n)
+        return n, time.perf_counter() - t0
+
+    n, dt = sample(seconds)
+    O.lib().orc_set_threads(1)
+    n1, dt1 = sample(seconds / 3.0)
+    O.lib().orc_set_threads(threads)
+    return {"value": round(nx * ny * nz * n / dt / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle, {n} steps of a 512x256x32 slab of the workload ({dt:.1f} s)",
+            "one_thread": {"value": round(nx * ny * nz * n1 / dt1 / 1e6, 2), "unit": "MLUPS",
+                           "sample": f"{n1} steps ({dt1:.1f} s)"},
+            "cpu_model": cpu_model(), "host_threads": os.cpu_count()}
 
 
 def cpu_reference_structured(seconds=10.0):
@@ -147,6 +188,40 @@ def adjoint_ratio(pkg):
     s.close()
     return {"workload": f"d3q27_adjoint_192x96x96 (fp32), J over steps {n0}-{n1}, checkpoint {W.checkpoint}",
             "forward_ms": round(tf, 3), "adjoint_ms": round(ta, 3), "ratio": round(ta / tf, 3), "J": J}
+
+
+def dropin_loop(pkg, steps=200, warm=10):
+    """The reference's own 3D driver loop through the drop-in classes (host/lbm_main on the
+    3D input.in case: D3Q27 512x128x128 sphere, reference boundary mode, FP64; Collision(),
+    pack/exchange/unpack, Streaming(), Update() per step, main.cpp:29-45) against one
+    lbm_step(n) call on the same case: seconds per step after `warm` steps, device drained at
+    both ends.  The run is capped at `steps` (--time), below the first snapshot step (inter =
+    Time / 10 = 3200, System.cpp:21-22), so only the initial snapshot is written, before timing."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(PKG_DIR, "host", "lbm_main")
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "input.in")
+        with open(path, "w") as fh:
+            fh.write("1.0  8.0  100\n0.1  100  10\n512  128  128\n")
+        r = subprocess.run([exe, "--input", path, "--time", str(steps), "--report", str(warm)], cwd=d,
+                           capture_output=True, text=True, timeout=300)
+        if r.returncode != 0:
+            raise RuntimeError(f"lbm_main failed: {r.stderr[-500:]}")
+        line = [l for l in r.stdout.splitlines() if l.startswith("loop_seconds")][-1].split()
+        loop_s, n = float(line[1]), int(line[3])
+    c = pkg.lbm.default_config(27)
+    with pkg.Solver(c) as s:
+        s.initialize()
+        s.step(warm)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(n)
+        s.sync()
+        step_s = time.perf_counter() - t0
+    return {"workload": "3D input.in case: D3Q27 512x128x128 sphere, reference mode, fp64",
+            "dropin_loop_ms_per_step": round(loop_s / n * 1e3, 5), "lbm_step_ms_per_step": round(step_s / n * 1e3, 5),
+            "ratio": round(loop_s / step_s, 4), "steps": n}
 
 
 def config_legs(pkg):
@@ -341,7 +416,7 @@ def main():
         print(json.dumps({"adjoint": adjoint_ratio(pkg)}), flush=True)
         return
     if a.configs_only:
-        print(json.dumps({"configs": config_legs(pkg)}), flush=True)
+        print(json.dumps({"configs": config_legs(pkg), "dropin_loop": dropin_loop(pkg)}), flush=True)
         return
 
     log(f"fp16s leg: {K} steps after {Wm} warm-up")
@@ -389,6 +464,8 @@ def main():
     if not a.no_configs:
         log("config 0 / config 1 legs")
         out["configs"] = config_legs(pkg)
+        log("drop-in loop")
+        out["dropin_loop"] = dropin_loop(pkg)
     if not a.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)

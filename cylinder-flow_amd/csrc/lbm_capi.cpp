@@ -187,17 +187,18 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_ctx **out) {
     c->aligned_pull = std::getenv("LBM_ALIGNED_PULL") != nullptr;
     // Blocks of the packed kernel resident per CU (256-lane blocks, row-interleaved populations).
     // D3Q27, measured on the 512x256x256 cylinder: FP32 fastest at 3 (3 waves per SIMD: 71.5 %
-    // of HBM peak per step against 70.8 % uncapped), FP16/BF16 at 4 (69.9 % against 67.3 % for
-    // FP16).  D2Q9, measured on an 8192x4096 disk: FP32 fastest uncapped (78.5 % against 69.3 %
-    // at 3), FP16/BF16 at 6 (73.5 % against 71.7 % uncapped and 67.4 % at 4).  The cap is a
-    // dynamic LDS allocation the kernel does not use.  LBM_FAST_BLOCKS_PER_CU overrides it
-    // (0: no cap).  Rows of fewer than 256 lanes (narrow domains) keep the same waves per CU.
+    // of HBM peak per step against 70.8 % uncapped), FP16/BF16 at 5 (steps 6-25 of the seeded
+    // start: 0.647 ms per step against 0.664 at 4 and 0.649 at 6; later steps equal at 4 and 5).
+    // D2Q9, measured on an 8192x4096 disk: FP32 fastest uncapped (78.5 % against 69.3 % at 3),
+    // FP16/BF16 at 6 (73.5 % against 71.7 % uncapped and 67.4 % at 4).  The cap is a dynamic LDS
+    // allocation the kernel does not use.  LBM_FAST_BLOCKS_PER_CU overrides it (0: no cap).
+    // Rows of fewer than 256 lanes (narrow domains) keep the same waves per CU.
     // Reference mode's fused kernel (FP64, D3Q27): fastest at 3 (512x128x128 sphere: 0.730 ms per
     // step against 0.767 uncapped, 0.761 at 2, 0.751 at 4); the plain FP64 kernel gains nothing
     // (0.628-0.630 ms at none, 3, 4; 0.701 at 2), so it stays uncapped.
     int bpc = 0;
     if (c->prec == LBM_FP32) bpc = Q == 27 ? 3 : 0;
-    else if (c->prec != LBM_FP64) bpc = Q == 27 ? 4 : 6;
+    else if (c->prec != LBM_FP64) bpc = Q == 27 ? 5 : 6;
     else if (ref_mode(cfg->bc_mode) && Q == 27) bpc = 3;
     if (const char *env = std::getenv("LBM_FAST_BLOCKS_PER_CU")) bpc = std::max(0, std::atoi(env));
     hipError_t e = hipSuccess;

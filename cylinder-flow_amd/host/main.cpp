@@ -9,7 +9,8 @@
 //   variant's parameters (Time 100000, inter 10000), sx+1 by sy+1 nodes, its wall and inlet
 //   fill order, no drag, Output(0) and Output every `inter` steps.
 // Extra flags (not in the reference): --input PATH, --time N (cap the step count),
-//   --check (parse and print the parameters, no GPU work).
+//   --check (parse and print the parameters, no GPU work), --report W (3D: print the loop time
+//   of the steps after step W, the device drained at both ends, as "loop_seconds S steps N").
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -55,7 +56,7 @@ static LBMOptions options_from(const System &s) {
 
 int main(int argc, char **argv) {
     std::string input = "input.in";
-    long cap = -1;
+    long cap = -1, report = -1;
     bool check = false, two_d = false, single = false, staged = false;
     for (int a = 1; a < argc; a++) {
         if (!std::strcmp(argv[a], "--input") && a + 1 < argc) input = argv[++a];
@@ -64,8 +65,10 @@ int main(int argc, char **argv) {
         else if (!std::strcmp(argv[a], "--2d")) two_d = true;
         else if (!std::strcmp(argv[a], "--2d-single")) two_d = single = true;
         else if (!std::strcmp(argv[a], "--staged")) staged = true;
+        else if (!std::strcmp(argv[a], "--report") && a + 1 < argc) report = std::atol(argv[++a]);
         else {
-            std::fprintf(stderr, "usage: %s [--input input.in] [--time N] [--check] [--2d | --2d-single] [--staged]\n",
+            std::fprintf(stderr,
+                         "usage: %s [--input input.in] [--time N] [--check] [--2d | --2d-single] [--staged] [--report W]\n",
                          argv[0]);
             return 2;
         }
@@ -112,6 +115,7 @@ int main(int argc, char **argv) {
         l1.setup_subdomain();
         const auto start = std::chrono::steady_clock::now();
         const long T = cap >= 0 && cap < s1.Time ? cap : s1.Time;
+        auto t_report = start;
         for (long it = 1; it <= T; it++) {
             l1.Collision();
             l1.pack();
@@ -124,11 +128,23 @@ int main(int argc, char **argv) {
                 const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
                 std::printf("time=%f\n", t);
             }
+            if (it == report) {
+                if (lbm_sync(l1.ctx) != LBM_OK) {
+                    std::fprintf(stderr, "%s\n", lbm_last_error(l1.ctx));
+                    return 1;
+                }
+                t_report = std::chrono::steady_clock::now();
+            }
         }
-        if (lbm_sync(l1.ctx) != LBM_OK || lbm_snapshot_wait(l1.ctx) != LBM_OK) {
+        const int rc_sync = lbm_sync(l1.ctx);
+        const auto t_end = std::chrono::steady_clock::now();   // before joining the snapshot writer
+        if (rc_sync != LBM_OK || lbm_snapshot_wait(l1.ctx) != LBM_OK) {
             std::fprintf(stderr, "%s\n", lbm_last_error(l1.ctx));
             return 1;
         }
+        if (report >= 0 && report < T)
+            std::printf("loop_seconds %.6f steps %ld\n", std::chrono::duration<double>(t_end - t_report).count(),
+                        T - report);
     } catch (const std::exception &e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 1;

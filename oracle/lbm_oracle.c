@@ -43,6 +43,15 @@ int orc_num_threads(void)
 #endif
 }
 
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 /* ------------------------------------------------------------------------------------ */
 /* Lattice tables in the reference order.                                                */
 /*   D2Q9 : MPI Kokkos/lbm.cpp:52-90      D3Q27: 3D MPI+KOKKOS+MPIIO/lbm.cpp:55-218       */

@@ -109,6 +109,7 @@ int      orc_plain_forces(const orc_plain *s, double *out /* 4 per record */, in
 /* one step's pre-collision populations g (after streaming + boundaries), SoA */
 void     orc_plain_last_g(const orc_plain *s, double *out);
 int      orc_num_threads(void);
+void     orc_set_threads(int n);   /* OpenMP threads of the following calls (n > 0) */
 
 /* ---- adjoint of the plain-mode step (FP64, partial bounce-back body) ---- */
 /* J = mean over steps n in [n0, n1] (1-based) of cd_scale * Fx_n, where Fx_n is the

@@ -85,6 +85,7 @@ def lib():
         L.orc_plain_forces.restype = C.c_int
         L.orc_plain_forces.argtypes = [P, P, C.c_int]
         L.orc_num_threads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
         L.orc_adjoint.restype = C.c_double
         L.orc_adjoint.argtypes = [C.POINTER(OrcCfg), P, P, P, P, C.c_int, C.c_int, C.c_int,
                                   C.c_double, C.c_int, P, P]

@@ -365,7 +365,7 @@ PLANES = ("LBM_DIRECTION_PLANES", (None, "1"))
     ("bf16s", "LBM_ALIGNED_PULL", (None, "1"), False),
     ("fp16s", "LBM_ALIGNED_PULL", (None, "1"), True),
     ("fp32", "LBM_FAST_BLOCKS_PER_CU", ("3", "0"), False),
-    ("fp16s", "LBM_FAST_BLOCKS_PER_CU", ("4", "0"), False),
+    ("fp16s", "LBM_FAST_BLOCKS_PER_CU", ("5", "0"), False),
     ("fp64", *PLANES, False), ("fp32", *PLANES, False), ("fp16s", *PLANES, False), ("fp16s", *PLANES, True)])
 def test_launch_variants_give_identical_results(pkg, monkeypatch, prec, env, values, periodic):
     """The aligned 16-bit pull (aligned loads, lane shifts, LDS hand-over at wave edges) equals
