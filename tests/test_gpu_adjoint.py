@@ -100,7 +100,8 @@ def test_adjoint2d_matches_finite_differences(pkg):
               objective(pkg, W.cfg, n0, n1, W.cd_scale(), sfrac=sm)) / (2 * eps)
         errs.append(abs(ds[0, y, x] - fd) / max(abs(fd), 1e-3 * np.abs(ds).max()))
     print(f"[adjoint] 2D cylinder dJ/ds vs FD on 32 nodes: max rel err {max(errs):.2e}")
-    assert max(errs) <= 1e-4
+    # observed 6.8e-7 (FP64 adjoint against FP64 central differences)
+    assert max(errs) <= 3e-6
     # uniform inlet velocity: dJ/dU = sum over inlet nodes of dJ/du_in
     c = L.LbmConfig.from_buffer_copy(W.cfg)
     vals = []
@@ -109,7 +110,7 @@ def test_adjoint2d_matches_finite_differences(pkg):
         vals.append(objective(pkg, c, n0, n1, W.cd_scale(), sfrac=sf))
     fd = (vals[0] - vals[1]) / (2 * eps)
     print(f"[adjoint] 2D cylinder dJ/du_in: adjoint {du.sum():.8e} FD {fd:.8e}")
-    assert abs(du.sum() - fd) <= 1e-4 * abs(fd)
+    assert abs(du.sum() - fd) <= 1e-6 * abs(fd)
 
 
 def test_adjoint3d_coefficients_and_cost(pkg):
@@ -168,9 +169,12 @@ def test_adjoint3d_coefficients_and_cost(pkg):
           f"[adjoint] config4 dJ/da vs FD (max rel to max|FD|): fp64 {err64.max():.2e}, fp32 {err32.max():.2e}\n"
           f"[adjoint] config4 dJ/ds vs FD at 8 nodes: fp64 {es64.max():.2e}, fp32 {es32.max():.2e}; "
           f"fp32 vs fp64 dJ/ds rel-L2 {rel_l2(ds32, ds64):.2e}", flush=True)
-    assert err64.max() <= 1e-4 and es64.max() <= 1e-4
-    assert err32.max() <= 5e-2
-    assert rel_l2(ds32, ds64) <= 5e-2
+    # observed: dJ/da fp64 1.3e-12, fp32 2.8e-6; dJ/ds fp64 2.9e-5, fp32 5.5e-5 (the FD's own
+    # truncation and rounding); fp32 against fp64 dJ/ds 8.1e-6.  Bounds about three times those.
+    assert err64.max() <= 1e-9 and es64.max() <= 1e-4
+    assert err32.max() <= 1e-5
+    assert es32.max() <= 2e-4
+    assert rel_l2(ds32, ds64) <= 3e-5
 
 
 def test_reduced_adjoint3d_matches_cpu_fp64_adjoint(pkg, orc):
@@ -187,7 +191,8 @@ def test_reduced_adjoint3d_matches_cpu_fp64_adjoint(pkg, orc):
     print("[adjoint] reduced 3D sphere CPU FP64 reference", flush=True)
     Jo, dso, duo = orc.adjoint(o, sfrac=sf, n0=n0, n1=n1, scale=scale, K=K)
     phi = legendre_basis(ny, nz).numpy()
-    for prec, tol in (("fp64", 1e-9), ("fp32", 5e-3)):
+    # observed worst component: fp64 9e-16, fp32 1.3e-5
+    for prec, tol in (("fp64", 1e-12), ("fp32", 5e-5)):
         g = L.make_config(27, nx, ny, nz, tau0=3 * 0.05 * D / 40.0, u0=0.05, precision=L.PRECISIONS[prec],
                           inlet=L.INLET_LEGENDRE, coef=tuple(coef), perturb=1e-3)
         with L.Solver(g) as s:
