@@ -21,6 +21,30 @@
 
 namespace lbm {
 
+// a*b + c rounded once (FP32 or FP64)
+template <class C>
+__device__ __forceinline__ C fmac(C a, C b, C c) {
+    if constexpr (sizeof(C) == 4) return __builtin_fmaf(a, b, c);
+    else return __builtin_fma(a, b, c);
+}
+
+// e_i . u with the zero components of e_i skipped
+template <class L, int i, class C>
+__device__ __forceinline__ C edot(const C *u) {
+    C r = 0;
+    bool first = true;
+    static_for<0, 3>([&](auto K) {
+        constexpr int k = K;
+        constexpr int e = k == 0 ? L::ex(i) : (k == 1 ? L::ey(i) : L::ez(i));
+        if constexpr (e != 0) {
+            const C t = e > 0 ? u[k] : -u[k];
+            r = first ? t : r + t;
+            first = false;
+        }
+    });
+    return r;
+}
+
 // Transposes of the inlet's regularisation and Zou-He completion applied to m = mu_m at an inlet
 // node (x = 0, x not periodic), in place, and their contribution to dJ/du_in of its row.
 template <int Q, class S>
@@ -78,6 +102,140 @@ __device__ __forceinline__ void adj_inlet_transpose(typename S::C *m, double &du
     });
 }
 
+// index of the direction with components (ex, ey, ez)
+template <class L>
+__host__ __device__ constexpr int dir_of(int ex, int ey, int ez) {
+    for (int i = 0; i < (L::D == 3 ? 27 : 9); i++)
+        if (L::ex(i) == ex && L::ey(i) == ey && L::ez(i) == ez) return i;
+    return -1;
+}
+
+// Sums over the lattice taken one axis at a time: a line of three values v(-1), v(0), v(+1)
+// along an axis becomes v(+1) + v(-1) + c v(0) (the sum, with the weight ratio c), v(+1) - v(-1)
+// (first moment) and v(+1) + v(-1) (second moment).
+//
+// moments01: rho = sum g, j = sum g e (c = 1).
+template <class L, class C>
+__device__ __forceinline__ void moments01(const C *g, C &rho, C *j) {
+    constexpr int NZ = L::D == 3 ? 3 : 1;
+    C a0[3], a1[3], b0[3];
+    static_for<0, NZ>([&](auto ZZ) {
+        constexpr int ez = L::D == 3 ? (int)ZZ - 1 : 0;
+        C s0[3], s1[3];
+        static_for<0, 3>([&](auto YY) {
+            constexpr int ey = (int)YY - 1;
+            const C fp = g[dir_of<L>(1, ey, ez)], fm = g[dir_of<L>(-1, ey, ez)];
+            s0[YY] = (fp + fm) + g[dir_of<L>(0, ey, ez)];
+            s1[YY] = fp - fm;
+        });
+        a0[ZZ] = (s0[2] + s0[0]) + s0[1];
+        a1[ZZ] = s0[2] - s0[0];
+        b0[ZZ] = (s1[2] + s1[0]) + s1[1];
+    });
+    if constexpr (L::D == 3) {
+        rho = (a0[2] + a0[0]) + a0[1];
+        j[0] = (b0[2] + b0[0]) + b0[1];
+        j[1] = (a1[2] + a1[0]) + a1[1];
+        j[2] = a0[2] - a0[0];
+    } else {
+        rho = a0[0];
+        j[0] = b0[0];
+        j[1] = a1[0];
+        j[2] = 0;
+    }
+}
+
+// the product form of the weights that wmoments relies on
+template <class L>
+constexpr bool weights_factor() {
+    for (int i = 0; i < (L::D == 3 ? 27 : 9); i++) {
+        double p = 1.0;
+        const int e[3] = {L::ex(i), L::ey(i), L::ez(i)};
+        for (int a = 0; a < L::D; a++) p *= e[a] == 0 ? 4.0 / 6.0 : 1.0 / 6.0;
+        const double d = p - L::w(i);
+        if (d > 1e-15 || d < -1e-15) return false;
+    }
+    return true;
+}
+static_assert(weights_factor<Lat<9>>() && weights_factor<Lat<27>>(), "lattice weights are products over the axes");
+
+// wmoments: moments of w_i f_i to second order, M0 = sum w f, M1 = sum w f e, M2 = sum w f e e^T
+// as (xx, yy, zz, xy, xz, yz).  The weights are products over the axes, w(e) = prod_a w1(e_a) with
+// w1(0) = 4/6 and w1(+-1) = 1/6, so each axis pass uses c = 4 and the factor 6^-D comes last.
+template <class L, class C>
+__device__ __forceinline__ void wmoments(const C *f, C &M0, C *M1, C *M2) {
+    constexpr int NZ = L::D == 3 ? 3 : 1;
+    C a0[3], a1[3], a2[3], b0[3], b1[3], c0[3];
+    static_for<0, NZ>([&](auto ZZ) {
+        constexpr int ez = L::D == 3 ? (int)ZZ - 1 : 0;
+        C s0[3], s1[3], s2[3];
+        static_for<0, 3>([&](auto YY) {
+            constexpr int ey = (int)YY - 1;
+            const C fp = f[dir_of<L>(1, ey, ez)], fm = f[dir_of<L>(-1, ey, ez)];
+            const C t = fp + fm;
+            s0[YY] = fmac((C)4, f[dir_of<L>(0, ey, ez)], t);
+            s1[YY] = fp - fm;
+            s2[YY] = t;
+        });
+        const C t0 = s0[2] + s0[0], t1 = s1[2] + s1[0];
+        a0[ZZ] = fmac((C)4, s0[1], t0);
+        a1[ZZ] = s0[2] - s0[0];
+        a2[ZZ] = t0;
+        b0[ZZ] = fmac((C)4, s1[1], t1);
+        b1[ZZ] = s1[2] - s1[0];
+        c0[ZZ] = fmac((C)4, s2[1], s2[2] + s2[0]);
+    });
+    C r[10];
+    if constexpr (L::D == 3) {
+        const C t = a0[2] + a0[0];
+        r[0] = fmac((C)4, a0[1], t);                    // M0
+        r[1] = fmac((C)4, b0[1], b0[2] + b0[0]);        // x
+        r[2] = fmac((C)4, a1[1], a1[2] + a1[0]);        // y
+        r[3] = a0[2] - a0[0];                           // z
+        r[4] = fmac((C)4, c0[1], c0[2] + c0[0]);        // xx
+        r[5] = fmac((C)4, a2[1], a2[2] + a2[0]);        // yy
+        r[6] = t;                                       // zz
+        r[7] = fmac((C)4, b1[1], b1[2] + b1[0]);        // xy
+        r[8] = b0[2] - b0[0];                           // xz
+        r[9] = a1[2] - a1[0];                           // yz
+    } else {
+        r[0] = a0[0];
+        r[1] = b0[0];
+        r[2] = a1[0];
+        r[3] = 0;
+        r[4] = c0[0];
+        r[5] = a2[0];
+        r[6] = 0;
+        r[7] = b1[0];
+        r[8] = 0;
+        r[9] = 0;
+    }
+    constexpr double sc = L::D == 3 ? 1.0 / 216.0 : 1.0 / 36.0;
+    M0 = r[0] * (C)sc;
+    static_for<0, 3>([&](auto K) { M1[K] = r[1 + K] * (C)sc; });
+    static_for<0, 6>([&](auto K) { M2[K] = r[4 + K] * (C)sc; });
+}
+
+// Byte offsets, from a row's first element, of direction i's row segment that the fast path
+// reads: mu_{m+1} in row + (e_y + ny e_z) (the gather along +e_i) and H in row - (e_y + ny e_z)
+// (the pull along -e_i).
+struct AdjTab {
+    int64_t mun[27];
+    int64_t h[27];
+};
+
+template <int Q, class C, class T>
+static AdjTab adj_tab(const Geo &g) {
+    using L = Lat<Q>;
+    AdjTab t{};
+    for (int i = 0; i < Q; i++) {
+        const int64_t dr = (int64_t)L::ey(i) + (int64_t)g.ny * (int64_t)L::ez(i);
+        t.mun[i] = ((int64_t)i * (int64_t)g.ds + dr * (int64_t)g.rs) * (int64_t)sizeof(C);
+        t.h[i] = ((int64_t)i * (int64_t)g.ds - dr * (int64_t)g.rs) * (int64_t)sizeof(T);
+    }
+    return t;
+}
+
 // Interior rows (no wall or periodic wrap in y/z) whose nodes are all fluid and have no solid
 // neighbour ("clean", flagged per row on the host) take a straight-line path: every source and
 // destination row of a direction is a scalar of the block, and the x edges (inlet, outlet,
@@ -89,11 +247,14 @@ __global__ void __launch_bounds__(256)
     k_adj_step(const typename S::T *__restrict__ H, const typename S::C *__restrict__ mun,
                typename S::C *__restrict__ mu, const uint8_t *__restrict__ flags, const typename S::C *__restrict__ sfrac,
                const double *__restrict__ uin, double *__restrict__ dJds, double *__restrict__ dJdu,
-               const uint8_t *__restrict__ rowclean, const AdjArgs a) {
+               const uint8_t *__restrict__ rowclean, const AdjArgs a, const AdjTab tab) {
     using L = Lat<Q>;
     using C = typename S::C;
     const Geo &g = a.f.g;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+    // the plane z = nz - 1 first: its rows take the general path (at a wall or periodic), whose
+    // dependent loads make its blocks the slowest; dispatched last they would form the tail
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    const int z = blockIdx.z == 0 ? g.nz - 1 : (int)blockIdx.z - 1;
     if (x >= g.nx) return;
     const uint32_t row = (uint32_t)y + (uint32_t)g.ny * (uint32_t)z;
     const uint32_t node = (uint32_t)x + (uint32_t)g.nxp * row;
@@ -110,55 +271,51 @@ __global__ void __launch_bounds__(256)
     C gv[Q];
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (fast) {
+        // Every load of the path is issued before any value is used, from the scalar base of
+        // its direction's row plus one of three lane offsets (x - 1, x, x + 1 after the x edge
+        // rule), so the 2Q + Q/3 loads are in flight together.
         // ---- 1. lambda: entry (node, opp i) pulls from node + e_i, entry (d, i) from node;
         //         the outlet node also feeds (nx-1, y + e_y, z + e_z) along e_ix < 0 ----
-        static_for<0, Q>([&](auto I) {
-            constexpr int i = I;
-            constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i);
-            const C *mrow = mun + (size_t)i * g.ds + (size_t)(row + (uint32_t)(ey + g.ny * ez)) * g.rs;
-            int dx = x + ex;
-            bool live = true;
-            if constexpr (ex > 0) {
-                if (dx == g.nx) {
-                    if (g.xper) dx = 0;
-                    else live = false, dx = x;   // beyond the outlet: no entry
-                }
-            }
-            if constexpr (ex < 0) {
-                if (dx < 0) {
-                    if (g.xper) dx = g.nx - 1;
-                    else live = false, dx = x;   // (node, opp i) is a Zou-He unknown
-                }
-            }
-            const C v = mrow[dx];
-            lam[i] = live ? v : (C)0;
-        });
-        if (xhi) {   // the outlet lane only
-            static_for<0, Q>([&](auto I) {
-                constexpr int i = I;
-                constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i);
-                if constexpr (ex < 0)
-                    lam[i] += mun[(size_t)i * g.ds + (size_t)(row + (uint32_t)(ey + g.ny * ez)) * g.rs + (size_t)x];
-            });
-        }
         // ---- 2. recompute G_m(node): the forward pull (inlet unknowns 0, outlet clamp) ----
+        const uint32_t ux = (uint32_t)x;
+        const uint32_t xm = x > 0 ? ux - 1u : (g.xper ? (uint32_t)g.nx - 1u : ux);
+        const uint32_t xp = x + 1 < g.nx ? ux + 1u : (g.xper ? 0u : ux);
+        const bool xlive_m = x > 0 || g.xper, xlive_p = x + 1 < g.nx || g.xper;
+        const char *const Mrow = (const char *)mun + (size_t)row * g.rs * sizeof(C);
+        const char *const Hrow = (const char *)H + (size_t)row * g.rs * sizeof(typename S::T);
+        C rl[Q], rx[Q];
+        typename S::T rh[Q];
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
-            constexpr int ex = L::ex(i), ey = L::ey(i), ez = L::ez(i);
-            const typename S::T *hrow = H + (size_t)i * g.ds + (size_t)(row - (uint32_t)(ey + g.ny * ez)) * g.rs;
-            int sx = x - ex;
-            bool zh = false;
-            if constexpr (ex > 0) {
-                if (sx < 0) {
-                    if (g.xper) sx = g.nx - 1;
-                    else zh = true, sx = 0;
-                }
-            }
+            constexpr int ex = L::ex(i);
+            const uint32_t bo = (uint32_t)(ex > 0 ? xp : (ex < 0 ? xm : ux)) * (uint32_t)sizeof(C);
+            rl[i] = *(const C *)(Mrow + tab.mun[i] + bo);
+        });
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr int ex = L::ex(i);
+            const uint32_t bo = (uint32_t)(ex > 0 ? xm : (ex < 0 ? xp : ux)) * (uint32_t)sizeof(typename S::T);
+            rh[i] = *(const typename S::T *)(Hrow + tab.h[i] + bo);
+        });
+        static_for<0, Q>([&](auto I) {   // the outlet term's row segment at the lane's own x
+            constexpr int i = I;
+            if constexpr (L::ex(i) < 0) rx[i] = *(const C *)(Mrow + tab.mun[i] + ux * (uint32_t)sizeof(C));
+        });
+        // keep the selects below after the last load: scheduled among the loads they make the
+        // wave wait for the first loads before it issues the rest
+        __builtin_amdgcn_sched_barrier(0);
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr int ex = L::ex(i);
+            if constexpr (ex > 0) lam[i] = xlive_p ? rl[i] : (C)0;
             if constexpr (ex < 0) {
-                if (sx == g.nx) sx = g.xper ? 0 : g.nx - 1;
+                const C l = xlive_m ? rl[i] : (C)0;
+                lam[i] = xhi ? l + rx[i] : l;
             }
-            const C v = S::template decode<i, L>(hrow[sx], a.f.sp.off[i]);
-            gv[i] = zh ? (C)0 : v;
+            if constexpr (ex == 0) lam[i] = rl[i];
+            const C v = S::template decode<i, L>(rh[i], a.f.sp.off[i]);
+            if constexpr (ex > 0) gv[i] = xlo ? (C)0 : v;
+            else gv[i] = v;
         });
     } else {
     // ---- 1. lambda = dJ/dH_m(node) ----
@@ -229,56 +386,83 @@ __global__ void __launch_bounds__(256)
     if (inlet) velocity_inlet<Q, C>(gv, (C)uin[y + g.ny * z]);
 
     // ---- 3. local transpose ----
-    C p = 0, u[3] = {0, 0, 0};
-    static_for<0, Q>([&](auto I) {
-        constexpr int i = I;
-        p += gv[i];
-        if constexpr (L::ex(i) != 0) u[0] += (C)L::ex(i) * gv[i];
-        if constexpr (L::ey(i) != 0) u[1] += (C)L::ey(i) * gv[i];
-        if constexpr (L::ez(i) != 0) u[2] += (C)L::ez(i) * gv[i];
-    });
+    // With c_i = g_i + om (feq_i - g_i), feq_i = w_i (3p + 3 e_i.u + 4.5 (e_i.u)^2 - 1.5 u.u):
+    //   dJ/ds  += sum_i lam_i (g_opp(i) - c_i) + 2 alpha u_x
+    //   mu_j    = (1-om)(1-s) lam_j + om (Lp + e_j.Lu) + s lam_opp(j) + 2 alpha s e_jx
+    // where Lp = (1-s) A0 and Lu = (1-s) (3 A + 9 B u - 3 A0 u) are the transposes of the
+    // moments, written with the moments of w_i lam_i: A0 = sum w lam, A = sum w lam e,
+    // B = sum w lam e e^T (the per-direction sum Lu = sum (1-s) w lam (3 e + 9 (e.u) e - 3 u)
+    // regrouped).  Opposite directions share |e.u| and (e.u)^2.
+    C p, u[3];
+    moments01<L, C>(gv, p, u);
     p = p * (C)(1.0 / 3.0);
-    const C om = (C)(1.0 / a.f.tau);
+    const C om = (C)a.f.rtau;   // 1/tau rounded once, on the host
     C sf = 0;
     if constexpr (SFRAC) sf = sfrac[node];
     const C al = (C)a.alpha;
     if constexpr (SFRAC) {
-        // dJ/ds = sum_i lam_i (g_opp(i) - c_i) + alpha * 2 u_x
-        double ds = 2.0 * a.alpha * (double)u[0];
-        const C usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        const C usq = fmac(u[0], u[0], fmac(u[1], u[1], u[2] * u[2]));
+        const C X = fmac((C)3, p, (C)-1.5 * usq);
+        C acc0 = 0, acc1 = 0;
         static_for<0, Q>([&](auto I) {
-            constexpr int i = I;
-            const C eu = (C)L::ex(i) * u[0] + (C)L::ey(i) * u[1] + (C)L::ez(i) * u[2];
-            const C feq = (C)L::w(i) * ((C)3 * p + (C)3 * eu + (C)4.5 * eu * eu - (C)1.5 * usq);
-            const C c = gv[i] + om * (feq - gv[i]);
-            ds += (double)(lam[i] * (gv[L::opp(i)] - c));
+            constexpr int i = I, o = L::opp(i);
+            if constexpr (i == o) {
+                const C c = fmac(om, (C)L::w(i) * X - gv[i], gv[i]);
+                acc0 = fmac(lam[i], gv[i] - c, acc0);
+            } else if constexpr (i < o) {
+                const C eu = edot<L, i>(u);
+                const C wt = (C)L::w(i) * fmac((C)4.5 * eu, eu, X);
+                const C w3 = (C)(3.0 * L::w(i)) * eu;
+                const C ci = fmac(om, (wt + w3) - gv[i], gv[i]);
+                const C co = fmac(om, (wt - w3) - gv[o], gv[o]);
+                acc0 = fmac(lam[i], gv[o] - ci, acc0);
+                acc1 = fmac(lam[o], gv[i] - co, acc1);
+            }
         });
-        dJds[node] += ds;
+        dJds[node] += 2.0 * a.alpha * (double)u[0] + ((double)acc0 + (double)acc1);
     }
-    C Lp = 0, Lu[3] = {0, 0, 0};
-    static_for<0, Q>([&](auto I) {
-        constexpr int i = I;
-        const C li = ((C)1 - sf) * lam[i] * (C)L::w(i);
-        const C eu = (C)L::ex(i) * u[0] + (C)L::ey(i) * u[1] + (C)L::ez(i) * u[2];
-        Lp += li;
-        const C k9 = (C)9 * eu;
-        Lu[0] += li * ((C)3 * (C)L::ex(i) + k9 * (C)L::ex(i) - (C)3 * u[0]);
-        Lu[1] += li * ((C)3 * (C)L::ey(i) + k9 * (C)L::ey(i) - (C)3 * u[1]);
-        Lu[2] += li * ((C)3 * (C)L::ez(i) + k9 * (C)L::ez(i) - (C)3 * u[2]);
-    });
+    // moments of w lam (B: xx, yy, zz, xy, xz, yz)
+    C A0, A[3], B[6];
+    wmoments<L, C>(lam, A0, A, B);
+    const C ks = (C)1 - sf;
+    // V = om Lu (+ 2 alpha s along x), P = om Lp
+    const C t3 = (C)3 * A0;
+    const C Bu0 = fmac(B[0], u[0], fmac(B[3], u[1], B[4] * u[2]));
+    const C Bu1 = fmac(B[3], u[0], fmac(B[1], u[1], B[5] * u[2]));
+    const C Bu2 = fmac(B[4], u[0], fmac(B[5], u[1], B[2] * u[2]));
+    const C kom = ks * om;
+    C V[3];
+    V[0] = kom * fmac((C)9, Bu0, fmac((C)3, A[0], -t3 * u[0]));
+    V[1] = kom * fmac((C)9, Bu1, fmac((C)3, A[1], -t3 * u[1]));
+    V[2] = kom * fmac((C)9, Bu2, fmac((C)3, A[2], -t3 * u[2]));
+    if constexpr (SFRAC) V[0] = fmac((C)2 * al, sf, V[0]);
+    const C P = kom * A0;
+    const C c1 = ((C)1 - om) * ks;
     C m[Q];
     static_for<0, Q>([&](auto J) {
         constexpr int j = J;
-        const C eL = (C)L::ex(j) * Lu[0] + (C)L::ey(j) * Lu[1] + (C)L::ez(j) * Lu[2];
-        m[j] = ((C)1 - om) * ((C)1 - sf) * lam[j] + om * (Lp + eL) + sf * lam[L::opp(j)] +
-               al * (C)2 * sf * (C)L::ex(j);
+        // E = P + e_z V_z + e_y V_y + e_x V_x, in this order so that directions sharing
+        // (e_z) or (e_z, e_y) share the partial sums
+        C E = P;
+        if constexpr (L::ez(j) > 0) E = E + V[2];
+        if constexpr (L::ez(j) < 0) E = E - V[2];
+        if constexpr (L::ey(j) > 0) E = E + V[1];
+        if constexpr (L::ey(j) < 0) E = E - V[1];
+        if constexpr (L::ex(j) > 0) E = E + V[0];
+        if constexpr (L::ex(j) < 0) E = E - V[0];
+        if constexpr (SFRAC) m[j] = fmac(c1, lam[j], fmac(sf, lam[L::opp(j)], E));
+        else m[j] = fmac(c1, lam[j], E);
     });
     if (inlet) {
         double du = 0.0;
         adj_inlet_transpose<Q, S>(m, du);
         dJdu[y + (size_t)g.ny * z] += du;
     }
-    static_for<0, Q>([&](auto I) { __builtin_nontemporal_store(m[I], &mu[pel(g, I, (uint32_t)x, row)]); });
+    char *const murow = (char *)mu + (size_t)row * g.rs * sizeof(C);
+    const uint32_t bx = (uint32_t)x * (uint32_t)sizeof(C);
+    static_for<0, Q>([&](auto I) {
+        __builtin_nontemporal_store(m[I], (C *)(murow + (size_t)I * g.ds * sizeof(C) + bx));
+    });
 }
 
 
@@ -384,12 +568,13 @@ static hipError_t adj_step_qs(const void *H, const void *mun, void *mu, const ui
     // blocks of one row's lanes (at most 256), so the fast path's test is uniform per block
     const Shape sh = lane_shape(a.f.g, 1);
     const dim3 grid(sh.gx, a.f.g.ny, a.f.g.nz);
+    const AdjTab tab = adj_tab<Q, C, T>(a.f.g);
     if (sfrac)
         hipLaunchKernelGGL((k_adj_step<Q, S, true>), grid, dim3(sh.bx), 0, s, (const T *)H, (const C *)mun, (C *)mu, flags,
-                           (const C *)sfrac, uin, (double *)dJds, dJdu, rowclean, a);
+                           (const C *)sfrac, uin, (double *)dJds, dJdu, rowclean, a, tab);
     else
         hipLaunchKernelGGL((k_adj_step<Q, S, false>), grid, dim3(sh.bx), 0, s, (const T *)H, (const C *)mun, (C *)mu, flags,
-                           (const C *)nullptr, uin, (double *)dJds, dJdu, rowclean, a);
+                           (const C *)nullptr, uin, (double *)dJds, dJdu, rowclean, a, tab);
     return hipGetLastError();
 }
 

@@ -71,7 +71,7 @@ def kernel_indices(Q, nx, ny, nz, per, mask):
                         if ex > 0 and dx == nx: dx = 0 if xper else x
                         if ex < 0 and dx < 0: dx = nx - 1 if xper else x
                         idx.append(pel(i, dx, mrow))
-                        if ex < 0 and xhi: idx.append(pel(i, x, mrow))
+                        if ex < 0: idx.append(pel(i, x, mrow))     # the outlet term, loaded by every lane
                         hrow = (row - ((ey + ny * ez) % U32)) % U32
                         sx = x - ex
                         if ex > 0 and sx < 0: sx = nx - 1 if xper else 0
