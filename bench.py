@@ -120,7 +120,8 @@ def cpu_model():
 def cpu_baseline(seconds=15.0):
     """The FP64 CPU oracle (push-form port of the plain-mode step, OpenMP) on a slab of the same
     workload: 512x256x32 nodes with the same cylinder, walls, inlet and outlet, on all the
-    threads OpenMP is given and on one thread (a shorter sample)."""
+    threads OpenMP is given and on one thread (a shorter sample); then the same step built in
+    float (oracle/liboracle_f32.so) on all threads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     nx, ny, nz = 512, 256, 32
@@ -144,10 +145,26 @@ def cpu_baseline(seconds=15.0):
     O.lib().orc_set_threads(1)
     n1, dt1 = sample(seconds / 3.0)
     O.lib().orc_set_threads(threads)
+    # FP32: the float build of the same step, all threads
+    L32 = O.f32_plain_lib()
+    L32.orc_set_threads(threads)
+    import ctypes
+    h = L32.orc_plain_create(ctypes.byref(cfg), None, None, None, None)
+    L32.orc_plain_step(h, 1)
+    t0 = time.perf_counter()
+    L32.orc_plain_step(h, 1)
+    one = time.perf_counter() - t0
+    n32 = max(2, int(seconds / 2.0 / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    L32.orc_plain_step(h, n32)
+    dt32 = time.perf_counter() - t0
+    L32.orc_plain_destroy(h)
     return {"value": round(nx * ny * nz * n / dt / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
             "sample": f"FP64 oracle, {n} steps of a 512x256x32 slab of the workload ({dt:.1f} s)",
             "one_thread": {"value": round(nx * ny * nz * n1 / dt1 / 1e6, 2), "unit": "MLUPS",
                            "sample": f"{n1} steps ({dt1:.1f} s)"},
+            "fp32": {"value": round(nx * ny * nz * n32 / dt32 / 1e6, 2), "unit": "MLUPS", "cores": threads,
+                     "sample": f"the oracle's step built in float, {n32} steps of the same slab ({dt32:.1f} s)"},
             "cpu_model": cpu_model(), "host_threads": os.cpu_count()}
 
 

@@ -6,6 +6,10 @@
  *
  * Built with -ffp-contract=off so every expression rounds exactly as written; the
  * reference's pow(x, 2) is written x*x (both compilers fold pow(x, 2.0) to x*x).
+ *
+ * Built a second time with -DORC_PLAIN_FLOAT (liboracle_f32.so) for the CPU baseline's FP32
+ * leg only: the plain mode's populations and step arithmetic are then float, and the
+ * adjoint is left out.  The default build is the FP64 oracle.
  */
 #include "lbm_oracle.h"
 
@@ -14,6 +18,13 @@
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
+#endif
+
+/* the plain mode's population and step arithmetic type */
+#ifdef ORC_PLAIN_FLOAT
+typedef float preal;
+#else
+typedef double preal;
 #endif
 
 /* ------------------------------------------------------------------------------------ */
@@ -124,6 +135,23 @@ static double feq_ref(const lattice *L, int ii, double p, double u, double v, do
     double eu2 = edu * edu;
     return L->t[ii] * p * 3.0 + L->t[ii] * (3.0 * edu + 4.5 * eu2 - 1.5 * udu);
 }
+
+#ifdef ORC_PLAIN_FLOAT
+/* feq_ref's expression in the plain-mode type */
+static preal feq_p(const lattice *L, int ii, preal p, preal u, preal v, preal w)
+{
+    preal edu = (preal)L->e[ii][0] * u + (preal)L->e[ii][1] * v;
+    preal udu = u * u + v * v;
+    if (L->D == 3) {
+        edu = edu + (preal)L->e[ii][2] * w;
+        udu = udu + w * w;
+    }
+    preal eu2 = edu * edu;
+    return (preal)L->t[ii] * p * (preal)3 + (preal)L->t[ii] * ((preal)3 * edu + (preal)4.5 * eu2 - (preal)1.5 * udu);
+}
+#else
+#define feq_p feq_ref
+#endif
 
 int orc_feq(int Q, double p, double u, double v, double w, double *out)
 {
@@ -548,7 +576,7 @@ struct orc_plain {
     unsigned char *solid;
     double *s;                  /* solid fraction or NULL */
     double *uin;                /* ny*nz */
-    double *h, *g;              /* SoA [q][N] */
+    preal *h, *g;               /* SoA [q][N] */
     double *mp, *mu, *mv, *mw;
     double *rowF;               /* 3 per (y,z) row */
     long step;
@@ -634,8 +662,8 @@ orc_plain *orc_plain_create(const orc_cfg *c, const unsigned char *mask, const d
     const size_t N = s->N;
     s->solid = (unsigned char *)calloc(N, 1);
     s->uin = (double *)calloc((size_t)s->ny * s->nz, sizeof(double));
-    s->h = (double *)malloc(N * Q * sizeof(double));
-    s->g = (double *)malloc(N * Q * sizeof(double));
+    s->h = (preal *)malloc(N * Q * sizeof(preal));
+    s->g = (preal *)malloc(N * Q * sizeof(preal));
     s->mp = (double *)calloc(N, sizeof(double));
     s->mu = (double *)calloc(N, sizeof(double));
     s->mv = (double *)calloc(N, sizeof(double));
@@ -657,7 +685,7 @@ orc_plain *orc_plain_create(const orc_cfg *c, const unsigned char *mask, const d
         s->solid[n] = (unsigned char)sol;
         double u = 0.0;
         if (!sol) u = s->uin[y + (size_t)s->ny * z] + c->perturb * (xi ? xi[n] : orc_xi(c->seed, (uint64_t)n));
-        for (int ii = 0; ii < Q; ii++) s->h[ii * N + n] = sol ? 0.0 : feq_ref(&s->L, ii, 0.0, u, 0.0, 0.0);
+        for (int ii = 0; ii < Q; ii++) s->h[ii * N + n] = (preal)(sol ? 0.0 : feq_ref(&s->L, ii, 0.0, u, 0.0, 0.0));
         s->mu[n] = u;
     }
     return s;
@@ -673,9 +701,9 @@ void orc_plain_destroy(orc_plain *s)
 /* Zou-He velocity inlet for the shifted incompressible model: for each unknown i (e_ix>0)
  *   g_i = g_bb(i) + 6 w_i U - 18 w_i (e_iy N_y + e_iz N_z),  N_t = sum_{e_kx=0} g_k e_kt,
  * which reproduces sum_i g_i e_i = (U, 0, 0) exactly (no reference counterpart). */
-static void zou_he(const lattice *L, double *gv, double U)
+static void zou_he(const lattice *L, preal *gv, preal U)
 {
-    double Ny = 0.0, Nz = 0.0;
+    preal Ny = 0.0, Nz = 0.0;
     for (int k = 0; k < L->Q; k++) {
         if (L->e[k][0] != 0) continue;
         if (L->e[k][1] > 0) Ny += gv[k]; else if (L->e[k][1] < 0) Ny -= gv[k];
@@ -683,10 +711,10 @@ static void zou_he(const lattice *L, double *gv, double U)
     }
     for (int i = 0; i < L->Q; i++) {
         if (L->e[i][0] <= 0) continue;
-        double tr = 0.0;
+        preal tr = 0.0;
         if (L->e[i][1] > 0) tr += Ny; else if (L->e[i][1] < 0) tr -= Ny;
         if (L->e[i][2] > 0) tr += Nz; else if (L->e[i][2] < 0) tr -= Nz;
-        gv[i] = (gv[L->bb[i]] + (6.0 * L->t[i]) * U) - (18.0 * L->t[i]) * tr;
+        gv[i] = (gv[L->bb[i]] + (preal)(6.0 * L->t[i]) * U) - (preal)(18.0 * L->t[i]) * tr;
     }
 }
 
@@ -695,9 +723,9 @@ static void zou_he(const lattice *L, double *gv, double U)
  * populations,  g_i = w_i (S + 3 e_ix U + 4.5 Q_i:P),  Q_i = e_i e_i - delta/3.  The sums run
  * in direction order and every product is written out, so the FP64 GPU path (built, like
  * this file, without FMA contraction) reproduces it bit for bit. */
-static void regularize_inlet(const lattice *L, double *g, double U)
+static void regularize_inlet(const lattice *L, preal *g, preal U)
 {
-    double S = 0.0, Mxx = 0.0, Myy = 0.0, Mzz = 0.0, Mxy = 0.0, Mxz = 0.0, Myz = 0.0;
+    preal S = 0.0, Mxx = 0.0, Myy = 0.0, Mzz = 0.0, Mxy = 0.0, Mxz = 0.0, Myz = 0.0;
     for (int k = 0; k < L->Q; k++) {
         const int ex = L->e[k][0], ey = L->e[k][1], ez = L->e[k][2];
         S = S + g[k];
@@ -708,15 +736,15 @@ static void regularize_inlet(const lattice *L, double *g, double U)
         if (ex * ez > 0) Mxz = Mxz + g[k]; else if (ex * ez < 0) Mxz = Mxz - g[k];
         if (ey * ez > 0) Myz = Myz + g[k]; else if (ey * ez < 0) Myz = Myz - g[k];
     }
-    const double S3 = S * (1.0 / 3.0);
-    const double Pxx = Mxx - S3, Pyy = Myy - S3, Pzz = Mzz - S3;
-    double tr = Pxx + Pyy;
+    const preal S3 = S * (preal)(1.0 / 3.0);
+    const preal Pxx = Mxx - S3, Pyy = Myy - S3, Pzz = Mzz - S3;
+    preal tr = Pxx + Pyy;
     if (L->D == 3) tr = tr + Pzz;
-    const double d = tr * (1.0 / 3.0);
-    const double U3 = 3.0 * U;
+    const preal d = tr * (preal)(1.0 / 3.0);
+    const preal U3 = (preal)3 * U;
     for (int i = 0; i < L->Q; i++) {
         const int ex = L->e[i][0], ey = L->e[i][1], ez = L->e[i][2];
-        double q = 0.0, o = 0.0;
+        preal q = 0.0, o = 0.0;
         if (ex != 0) q = q + Pxx;
         if (ey != 0) q = q + Pyy;
         if (ez != 0) q = q + Pzz;
@@ -724,9 +752,9 @@ static void regularize_inlet(const lattice *L, double *g, double U)
         if (ex * ey > 0) o = o + Mxy; else if (ex * ey < 0) o = o - Mxy;
         if (ex * ez > 0) o = o + Mxz; else if (ex * ez < 0) o = o - Mxz;
         if (ey * ez > 0) o = o + Myz; else if (ey * ez < 0) o = o - Myz;
-        double t = S;
+        preal t = S;
         if (ex > 0) t = S + U3; else if (ex < 0) t = S - U3;
-        g[i] = L->t[i] * (t + (4.5 * q + 9.0 * o));
+        g[i] = (preal)L->t[i] * (t + ((preal)4.5 * q + (preal)9 * o));
     }
 }
 
@@ -748,7 +776,7 @@ void orc_plain_step(orc_plain *s, int nsteps)
                 if (s->solid[a]) continue;
                 for (int i = 0; i < Q; i++) {
                     const int *e = L->e[i];
-                    const double v = s->h[i * N + a];
+                    const preal v = s->h[i * N + a];
                     int dx = x + e[0], dy = y + e[1], dz = z + e[2];
                     int in = 1;
                     if (s->xper) dx = wrapi(dx, nx); else if (dx < 0 || dx >= nx) in = 0;
@@ -785,10 +813,10 @@ void orc_plain_step(orc_plain *s, int nsteps)
                     }
                 d = (size_t)nx * r;
                 if (!s->solid[d]) {
-                    double gv[27];
+                    preal gv[27];
                     for (int i = 0; i < Q; i++) gv[i] = s->g[i * N + d];
-                    zou_he(L, gv, s->uin[r]);
-                    regularize_inlet(L, gv, s->uin[r]);
+                    zou_he(L, gv, (preal)s->uin[r]);
+                    regularize_inlet(L, gv, (preal)s->uin[r]);
                     for (int i = 0; i < Q; i++) s->g[i * N + d] = gv[i];
                 }
             }
@@ -804,27 +832,27 @@ void orc_plain_step(orc_plain *s, int nsteps)
                     s->mp[a] = s->mu[a] = s->mv[a] = s->mw[a] = 0.0;
                     continue;
                 }
-                double gv[27];
-                double p = 0.0, u = 0.0, v = 0.0, w = 0.0;
+                preal gv[27];
+                preal p = 0.0, u = 0.0, v = 0.0, w = 0.0;
                 for (int i = 0; i < Q; i++) {
                     gv[i] = s->g[i * N + a];
-                    p = p + gv[i] / 3.0;
+                    p = p + gv[i] / (preal)3;
                     const int *e = L->e[i];
                     if (e[0] > 0) u = u + gv[i]; else if (e[0] < 0) u = u - gv[i];
                     if (e[1] > 0) v = v + gv[i]; else if (e[1] < 0) v = v - gv[i];
                     if (e[2] > 0) w = w + gv[i]; else if (e[2] < 0) w = w - gv[i];
                 }
                 s->mp[a] = p; s->mu[a] = u; s->mv[a] = v; s->mw[a] = w;
-                const double sf = s->s ? s->s[a] : 0.0;
+                const preal sf = (preal)(s->s ? s->s[a] : 0.0);
                 if (s->s) {
                     F[0] += 2.0 * sf * u;
                     F[1] += 2.0 * sf * v;
                     F[2] += 2.0 * sf * w;
                 }
                 for (int i = 0; i < Q; i++) {
-                    double feq = feq_ref(L, i, p, u, v, w);
-                    double c = gv[i] - (gv[i] - feq) / (s->tau0 + 0.5);
-                    if (s->s) c = (1.0 - sf) * c + sf * gv[L->bb[i]];
+                    preal feq = feq_p(L, i, p, u, v, w);
+                    preal c = gv[i] - (gv[i] - feq) / (preal)(s->tau0 + 0.5);
+                    if (s->s) c = ((preal)1 - sf) * c + sf * gv[L->bb[i]];
                     s->h[i * N + a] = c;
                 }
             }
@@ -847,9 +875,18 @@ void orc_plain_fields(const orc_plain *s, double *p, double *u, double *v, doubl
     if (v) memcpy(v, s->mv, b);
     if (w) memcpy(w, s->mw, b);
 }
-void orc_plain_populations(const orc_plain *s, double *out) { memcpy(out, s->h, s->N * s->L.Q * sizeof(double)); }
-void orc_plain_set_populations(orc_plain *s, const double *in) { memcpy(s->h, in, s->N * s->L.Q * sizeof(double)); }
-void orc_plain_last_g(const orc_plain *s, double *out) { memcpy(out, s->g, s->N * s->L.Q * sizeof(double)); }
+void orc_plain_populations(const orc_plain *s, double *out)
+{
+    for (size_t k = 0; k < s->N * s->L.Q; k++) out[k] = s->h[k];
+}
+void orc_plain_set_populations(orc_plain *s, const double *in)
+{
+    for (size_t k = 0; k < s->N * s->L.Q; k++) s->h[k] = (preal)in[k];
+}
+void orc_plain_last_g(const orc_plain *s, double *out)
+{
+    for (size_t k = 0; k < s->N * s->L.Q; k++) out[k] = s->g[k];
+}
 void orc_plain_inlet(const orc_plain *s, double *uin) { memcpy(uin, s->uin, (size_t)s->ny * s->nz * sizeof(double)); }
 int orc_plain_forces(const orc_plain *s, double *out, int max)
 {
@@ -858,6 +895,7 @@ int orc_plain_forces(const orc_plain *s, double *out, int max)
     return s->nrec;
 }
 
+#ifndef ORC_PLAIN_FLOAT
 /* ==================================================================================== */
 /* Discrete adjoint of the plain-mode step (FP64).  No reference counterpart.            */
 /*                                                                                      */
@@ -1074,3 +1112,4 @@ double orc_adjoint(const orc_cfg *cfg, const unsigned char *mask, const double *
     orc_plain_destroy(s);
     return J;
 }
+#endif /* ORC_PLAIN_FLOAT */

@@ -41,6 +41,27 @@ def build() -> None:
     subprocess.run(["make", "-s", "-B", "-C", HERE, "liboracle.so"], check=True)
 
 
+def f32_plain_lib():
+    """The float build of the plain mode (liboracle_f32.so: populations and step arithmetic in
+    float, no adjoint), for the CPU baseline's FP32 leg.  Same create/step/fields interface."""
+    path = os.path.join(HERE, "liboracle_f32.so")
+    if not os.path.exists(path):
+        subprocess.run(["make", "-s", "-B", "-C", HERE, "liboracle_f32.so"], check=True)
+    L = C.CDLL(path)
+    P = C.c_void_p
+    L.orc_plain_create.restype = P
+    L.orc_plain_create.argtypes = [C.POINTER(OrcCfg), P, P, P, P]
+    L.orc_plain_destroy.argtypes = [P]
+    L.orc_plain_step.argtypes = [P, C.c_int]
+    L.orc_plain_fields.argtypes = [P, P, P, P, P]
+    L.orc_plain_populations.argtypes = [P, P]
+    L.orc_plain_forces.restype = C.c_int
+    L.orc_plain_forces.argtypes = [P, P, C.c_int]
+    L.orc_num_threads.restype = C.c_int
+    L.orc_set_threads.argtypes = [C.c_int]
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:

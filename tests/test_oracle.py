@@ -463,3 +463,34 @@ def test_cpu_adjoint_matches_finite_differences(orc, Q, shape):
         J2, ds2, du2 = orc.adjoint(cfg, mask=mask, sfrac=s, uin=uin, n0=n0, n1=n1, K=6)
         assert du2.shape == sh
         assert abs(du2[row] - fd) <= 1e-6 * max(abs(fd), 1e-3)
+
+
+# -------------------------------------------------------------------- float build (baseline)
+@pytest.mark.parametrize("Q,nx,ny,nz", [(9, 80, 24, 1), (27, 40, 16, 12)])
+def test_float_build_tracks_the_fp64_oracle(orc, Q, nx, ny, nz):
+    """liboracle_f32.so (the CPU baseline's FP32 leg) runs the same plain-mode step in float:
+    after 60 steps with walls, inlet, outlet and a body its populations stay within FP32
+    round-off of the FP64 oracle's, and differ from them (the arithmetic is float)."""
+    import ctypes as C
+    L32 = orc.f32_plain_lib()
+    cfg = orc.make_cfg(Q, nx, ny, nz, tau0=0.05, u0=0.05, body=orc.BODY_ZCYL if Q == 27 else orc.BODY_SPHERE,
+                       center=(12.0, ny / 2 - 0.5, 0.0), r2=16.0, perturb=1e-3, force_interval=10)
+    ref = orc.PlainState(cfg)
+    ref.step(60)
+    h = L32.orc_plain_create(C.byref(cfg), None, None, None, None)
+    assert h
+    try:
+        L32.orc_plain_step(h, 60)
+        f32 = np.zeros((Q, nz, ny, nx))
+        L32.orc_plain_populations(h, f32.ctypes.data_as(C.c_void_p))
+        n = L32.orc_plain_forces(h, None, 0)
+        F32 = np.zeros((n, 4))
+        L32.orc_plain_forces(h, F32.ctypes.data_as(C.c_void_p), n)
+    finally:
+        L32.orc_plain_destroy(h)
+    f64 = ref.populations()
+    e = rel_l2(f32, f64)
+    assert 1e-9 < e < 1e-5, e
+    F64 = ref.forces()
+    assert np.array_equal(F32[:, 0], F64[:, 0])
+    assert np.abs(F32[:, 1] - F64[:, 1]).max() < 2e-5 * np.abs(F64[:, 1]).max()   # observed 2.5e-6
