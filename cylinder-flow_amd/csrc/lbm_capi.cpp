@@ -212,8 +212,9 @@ extern "C" int lbm_create(const lbm_config *cfg, lbm_ctx **out) {
             return LBM_ERR_HIP;
         }
         // the count is of 4-wave (256-lane) blocks: rows narrower than 256 lanes launch smaller
-        // blocks, and proportionally more of them may stay resident
-        const int lanes = nxp / c->V, wpb = (std::min(lanes, 256) + 63) / 64;
+        // blocks, and proportionally more of them may stay resident (the plain kernels); the
+        // reference mode's fused kernel always launches 256-lane blocks
+        const int lanes = ref_mode(cfg->bc_mode) ? 256 : nxp / c->V, wpb = (std::min(lanes, 256) + 63) / 64;
         const int blocks = std::max(1, bpc * 4 / wpb);
         // midway between lds/(blocks+1) and lds/blocks, so allocation granularity cannot tip the count
         const unsigned per = (unsigned)(2LL * lds / (2 * blocks + 1)) / 512u * 512u;

@@ -16,6 +16,8 @@
 // than the per-row blocks of k_fused, the same value to rounding).
 #include "fused_common.hpp"
 
+#include <cstdlib>
+
 namespace lbm {
 
 template <int TB_T, bool ANY_FORCE>
@@ -144,7 +146,14 @@ __global__ void __launch_bounds__(256) k_tb_force(const double *__restrict__ par
     }
 }
 
-int tb_depth() { return 5; }
+// steps per launch: 5 (LBM_TB_DEPTH overrides, 1-10)
+int tb_depth() {
+    if (const char *e = std::getenv("LBM_TB_DEPTH")) {
+        const int k = std::atoi(e);
+        if (k >= 1 && k <= 10) return k;
+    }
+    return 5;
+}
 
 size_t tb_lds_bytes(int T) { return sizeof(double) * 2 * 9 * T * T; }
 
@@ -154,10 +163,15 @@ int tb_tiles(const Geo &g, int T, int K, int *tiles_x) {
     return *tiles_x * ((g.ny + W - 1) / W);
 }
 
-// the tile side for a domain: 24 when its tiles fit the CUs once, else 32
+// the tile side for a domain: 24 when its tiles fit the CUs once (and leave a core of at least
+// 2 x 2 nodes), else 32; LBM_TB_TILE=24 or 32 overrides where the core stays at least 2 x 2
 int tb_tile(const Geo &g, int K, int cus) {
+    if (const char *e = std::getenv("LBM_TB_TILE")) {
+        const int t = std::atoi(e);
+        if ((t == 24 || t == 32) && t - 2 * K >= 2) return t;
+    }
     int tx = 0;
-    return tb_tiles(g, 24, K, &tx) <= cus ? 24 : 32;
+    return 24 - 2 * K >= 2 && tb_tiles(g, 24, K, &tx) <= cus ? 24 : 32;
 }
 
 hipError_t tb_prepare() {
