@@ -99,8 +99,14 @@ size_t tb_lds_bytes(int T);
 int tb_tiles(const Geo &g, int T, int K, int *tiles_x);
 int tb_tile(const Geo &g, int K, int cus);
 hipError_t tb_prepare();
-hipError_t tb_steps(const void *A, void *B, const uint8_t *flags, const double *uin, double *partials, double *hist,
-                    int rec0, int *nrec, const FusedArgs &a, int T, int K, int steps, long step0, int fi, hipStream_t s);
+// tb_steps: `steps` (<= K) steps in one launch; the partials of its force records go to record
+// slots pend, pend + 1, ... of `partials` (tb_records() slots of ntiles * 3 doubles); *nrec = their
+// count.  tb_force_flush sums npend slots into hist[rec0 ...], the first at step step1.
+int tb_records(int K);
+hipError_t tb_steps(const void *A, void *B, const uint8_t *flags, const double *uin, double *partials, int pend,
+                    int *nrec, const FusedArgs &a, int T, int K, int steps, long step0, int fi, hipStream_t s);
+hipError_t tb_force_flush(const double *partials, const Geo &g, int T, int K, double *hist, int rec0, int npend,
+                          long step1, int fi, hipStream_t s);
 // sum of the nslots force partials into hist[rec] (chunks: force_chunks(nslots) * 3 doubles)
 int force_chunks(int nslots);
 hipError_t force_reduce(const double *partials, int nslots, double *chunks, double *hist, int rec, long step,
@@ -195,7 +201,7 @@ struct lbm_ctx {
     int nslots = 0;
     int tb_k = 0;                   // D2Q9 FP64 small domains: steps per temporally blocked launch (0: off)
     int tb_t = 0;                   // and its tile side
-    double *tb_partials = nullptr;  // their per-tile force partials
+    double *tb_partials = nullptr;  // their per-tile force partials (tb_records(tb_k) record slots)
     uint32_t *fix = nullptr;        // FP32/16-bit: nodes recomputed per node after the packed path
     uint32_t *fixlk = nullptr;      // their link words
     int nfix = 0;

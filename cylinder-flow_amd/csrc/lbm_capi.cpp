@@ -609,7 +609,7 @@ static int init_plain(lbm_ctx *c) {
                 c->tb_t = T;
                 int tiles_x = 0;
                 const int nt = tb_tiles(g, T, K, &tiles_x);
-                HIP_TRY(c, hipMalloc(&c->tb_partials, sizeof(double) * 3 * (size_t)nt * K));
+                HIP_TRY(c, hipMalloc(&c->tb_partials, sizeof(double) * 3 * (size_t)nt * tb_records(K)));
             }
         }
     }
@@ -805,12 +805,25 @@ static int step_n(lbm_ctx *c, int n, hipEvent_t *evs = nullptr) {
     const FusedArgs a = ctx_fused_args(c);
     // temporally blocked launches for all but the last step (which stays a one-step launch, so
     // its input buffer can serve the fields on request); not under per-kernel timing
+    // (their force records are summed together, when the partials' slots are full and before the
+    // one-step launches, which write records of their own)
     int left = n;
+    int pend = 0, rec0 = 0;
+    long step1 = 0;
     while (c->tb_k > 0 && !evs && left > 1) {
         const int kk = std::min(c->tb_k, left - 1);
+        if (pend + kk > tb_records(c->tb_k)) {
+            HIP_TRY(c, tb_force_flush(c->tb_partials, a.g, c->tb_t, c->tb_k, c->hist, rec0, pend, step1, fi, c->stream));
+            pend = 0;
+        }
+        if (pend == 0 && fi > 0) {
+            rec0 = c->nrec;
+            step1 = (c->step + fi) / fi * fi;   // the first force step after the current one
+        }
         int nr = 0;
-        HIP_TRY(c, tb_steps(c->A, c->B, c->flags, c->duin, c->tb_partials, c->hist, c->nrec, &nr, a, c->tb_t, c->tb_k, kk,
-                            c->step, fi, c->stream));
+        HIP_TRY(c, tb_steps(c->A, c->B, c->flags, c->duin, c->tb_partials, pend, &nr, a, c->tb_t, c->tb_k, kk, c->step, fi,
+                            c->stream));
+        pend += nr;
         c->nrec += nr;
         c->step += kk;
         left -= kk;
@@ -818,6 +831,7 @@ static int step_n(lbm_ctx *c, int n, hipEvent_t *evs = nullptr) {
         c->fields_fresh = false;
         c->replay_ok = false;
     }
+    HIP_TRY(c, tb_force_flush(c->tb_partials, a.g, c->tb_t, c->tb_k, c->hist, rec0, pend, step1, fi, c->stream));
     for (int k = n - left; k < n; k++) {
         c->step++;
         const bool force = fi > 0 && c->step % fi == 0;

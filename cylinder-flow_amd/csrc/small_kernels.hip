@@ -24,7 +24,7 @@ template <int TB_T, bool ANY_FORCE>
 __global__ void __launch_bounds__(1024)
     k_tb2d(const double *__restrict__ A, double *__restrict__ B, const uint8_t *__restrict__ flags,
            const double *__restrict__ uin, double *__restrict__ partials, const FusedArgs a, int K, int steps,
-           long step0, int fi, int tiles_x, int ntiles) {
+           long step0, int fi, int tiles_x, int ntiles, int pend) {
     using L = Lat<9>;
     constexpr int Q = 9;
     const Geo &g = a.g;
@@ -112,7 +112,8 @@ __global__ void __launch_bounds__(1024)
             __shared__ double red[3 * 16];
             block_sum3<16>(fx, fy, fz, red);
             if (t == 0) {
-                const int r = (int)(step / fi - (step0 + 1 + fi - 1) / fi);   // record index within the launch
+                // record slot: the records already pending, then the index within the launch
+                const int r = pend + (int)(step / fi - (step0 + 1 + fi - 1) / fi);
                 double *P = partials + 3 * ((size_t)r * ntiles + blockIdx.x);
                 P[0] = fx;
                 P[1] = fy;
@@ -189,18 +190,24 @@ hipError_t tb_prepare() {
     return e;
 }
 
-// `steps` (<= K) steps from A into B, the first being step0 + 1; the force records among them go
-// to hist[rec0 ...] (partials: K * ntiles * 3 doubles of scratch).  Returns the records written.
-hipError_t tb_steps(const void *A, void *B, const uint8_t *flags, const double *uin, double *partials, double *hist,
-                    int rec0, int *nrec, const FusedArgs &a, int T, int K, int steps, long step0, int fi, hipStream_t s) {
+// record slots of the partials buffer: the force records of several launches are summed by one
+// k_tb_force launch (a launch of its own costs about as much as a step)
+int tb_records(int K) { return K > 128 ? K : 128; }
+
+// `steps` (<= K) steps from A into B, the first being step0 + 1; the partials of the force records
+// among them go to record slots pend, pend + 1, ... (the caller keeps pend + steps within
+// tb_records(K) and sums them with tb_force_flush).  Returns the number of records.
+hipError_t tb_steps(const void *A, void *B, const uint8_t *flags, const double *uin, double *partials, int pend,
+                    int *nrec, const FusedArgs &a, int T, int K, int steps, long step0, int fi, hipStream_t s) {
     int tiles_x = 0;
     const int ntiles = tb_tiles(a.g, T, K, &tiles_x);
     const size_t lds = tb_lds_bytes(T);
     const int nforce = fi > 0 ? (int)((step0 + steps) / fi - step0 / fi) : 0;
     *nrec = nforce;
+    if (pend < 0 || pend + nforce > tb_records(K)) return hipErrorInvalidValue;
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(ntiles), dim3(T * T), lds, s, (const double *)A, (double *)B, flags, uin, partials, a,
-                           K, steps, step0, fi, tiles_x, ntiles);
+                           K, steps, step0, fi, tiles_x, ntiles, pend);
     };
     if (T == 24) {
         if (nforce > 0) go(k_tb2d<24, true>);
@@ -209,10 +216,16 @@ hipError_t tb_steps(const void *A, void *B, const uint8_t *flags, const double *
         if (nforce > 0) go(k_tb2d<32, true>);
         else go(k_tb2d<32, false>);
     }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess || nforce == 0) return e;
-    const long step1 = (step0 + fi) / fi * fi;   // the first force step after step0
-    hipLaunchKernelGGL(k_tb_force, dim3(nforce), dim3(256), 0, s, (const double *)partials, ntiles, hist, rec0, step1, fi);
+    return hipGetLastError();
+}
+
+hipError_t tb_force_flush(const double *partials, const Geo &g, int T, int K, double *hist, int rec0, int npend,
+                          long step1, int fi, hipStream_t s) {
+    if (npend <= 0) return hipSuccess;
+    if (npend > tb_records(K) || fi <= 0) return hipErrorInvalidValue;
+    int tiles_x = 0;
+    const int ntiles = tb_tiles(g, T, K, &tiles_x);
+    hipLaunchKernelGGL(k_tb_force, dim3(npend), dim3(256), 0, s, partials, ntiles, hist, rec0, step1, fi);
     return hipGetLastError();
 }
 
