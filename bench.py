@@ -146,7 +146,7 @@ def cpu_baseline(seconds=15.0):
     n1, dt1 = sample(seconds / 3.0)
     O.lib().orc_set_threads(threads)
     # FP32: the float build of the same step, all threads
-    L32 = O.f32_plain_lib()
+    L32 = O.f32_lib()
     L32.orc_set_threads(threads)
     import ctypes
     h = L32.orc_plain_create(ctypes.byref(cfg), None, None, None, None)
@@ -189,8 +189,24 @@ def cpu_reference_structured(seconds=10.0):
     t0 = time.perf_counter()
     st.step(n)
     dt = time.perf_counter() - t0
+    del st
+    # FP32: the float build of the same reference mode
+    import ctypes
+    L32 = O.f32_lib()
+    h = L32.orc_ref_create(ctypes.byref(cfg), None)
+    L32.orc_ref_step(h, 1)
+    t0 = time.perf_counter()
+    L32.orc_ref_step(h, 1)
+    one = time.perf_counter() - t0
+    n32 = max(2, int(seconds / 2.0 / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    L32.orc_ref_step(h, n32)
+    dt32 = time.perf_counter() - t0
+    L32.orc_ref_destroy(h)
     return {"value": round(nx * ny * nz * n / dt / 1e6, 2), "unit": "MLUPS", "cores": O.lib().orc_num_threads(),
-            "kind": "port", "sample": f"FP64 oracle reference mode, {n} steps of a 512x256x32 slab ({dt:.1f} s)"}
+            "kind": "port", "sample": f"FP64 oracle reference mode, {n} steps of a 512x256x32 slab ({dt:.1f} s)",
+            "fp32": {"value": round(nx * ny * nz * n32 / dt32 / 1e6, 2), "unit": "MLUPS",
+                     "sample": f"the reference mode built in float, {n32} steps of the same slab ({dt32:.1f} s)"}}
 
 
 def adjoint_ratio(pkg):

@@ -7,9 +7,9 @@
  * Built with -ffp-contract=off so every expression rounds exactly as written; the
  * reference's pow(x, 2) is written x*x (both compilers fold pow(x, 2.0) to x*x).
  *
- * Built a second time with -DORC_PLAIN_FLOAT (liboracle_f32.so) for the CPU baseline's FP32
- * leg only: the plain mode's populations and step arithmetic are then float, and the
- * adjoint is left out.  The default build is the FP64 oracle.
+ * Built a second time with -DORC_FLOAT (liboracle_f32.so) for the CPU baseline's FP32
+ * legs only: the populations, fields and step arithmetic of both modes are then float (force
+ * sums stay double), and the adjoint is left out.  The default build is the FP64 oracle.
  */
 #include "lbm_oracle.h"
 
@@ -20,11 +20,11 @@
 #include <omp.h>
 #endif
 
-/* the plain mode's population and step arithmetic type */
-#ifdef ORC_PLAIN_FLOAT
-typedef float preal;
+/* the population, field and step arithmetic type */
+#ifdef ORC_FLOAT
+typedef float oreal;
 #else
-typedef double preal;
+typedef double oreal;
 #endif
 
 /* ------------------------------------------------------------------------------------ */
@@ -136,18 +136,18 @@ static double feq_ref(const lattice *L, int ii, double p, double u, double v, do
     return L->t[ii] * p * 3.0 + L->t[ii] * (3.0 * edu + 4.5 * eu2 - 1.5 * udu);
 }
 
-#ifdef ORC_PLAIN_FLOAT
+#ifdef ORC_FLOAT
 /* feq_ref's expression in the plain-mode type */
-static preal feq_p(const lattice *L, int ii, preal p, preal u, preal v, preal w)
+static oreal feq_p(const lattice *L, int ii, oreal p, oreal u, oreal v, oreal w)
 {
-    preal edu = (preal)L->e[ii][0] * u + (preal)L->e[ii][1] * v;
-    preal udu = u * u + v * v;
+    oreal edu = (oreal)L->e[ii][0] * u + (oreal)L->e[ii][1] * v;
+    oreal udu = u * u + v * v;
     if (L->D == 3) {
-        edu = edu + (preal)L->e[ii][2] * w;
+        edu = edu + (oreal)L->e[ii][2] * w;
         udu = udu + w * w;
     }
-    preal eu2 = edu * edu;
-    return (preal)L->t[ii] * p * (preal)3 + (preal)L->t[ii] * ((preal)3 * edu + (preal)4.5 * eu2 - (preal)1.5 * udu);
+    oreal eu2 = edu * edu;
+    return (oreal)L->t[ii] * p * (oreal)3 + (oreal)L->t[ii] * ((oreal)3 * edu + (oreal)4.5 * eu2 - (oreal)1.5 * udu);
 }
 #else
 #define feq_p feq_ref
@@ -180,8 +180,8 @@ struct orc_ref {
     int g;                      /* ghost width: 3 (3D) / 2 (2D) */
     int glx, gly, glz, lx, ly, lz;
     double tau0, u0;
-    double *f, *ft;             /* AoS, ii fastest (Kokkos LayoutLeft of f(ii,i,j,k)) */
-    double *ua, *va, *wa, *p;
+    oreal *f, *ft;              /* AoS, ii fastest (Kokkos LayoutLeft of f(ii,i,j,k)) */
+    oreal *ua, *va, *wa, *p;
     int *usr, *ran;
     long step;
     int force_interval;
@@ -228,12 +228,12 @@ orc_ref *orc_ref_create(const orc_cfg *c, const int *ran_in)
     s->force_interval = c->force_interval;
     s->variant = D == 2 ? c->variant : 0;
     size_t n = (size_t)s->lx * s->ly * s->lz;
-    s->f = (double *)calloc(n * Q, sizeof(double));
-    s->ft = (double *)calloc(n * Q, sizeof(double));
-    s->ua = (double *)calloc(n, sizeof(double));
-    s->va = (double *)calloc(n, sizeof(double));
-    s->wa = (double *)calloc(n, sizeof(double));
-    s->p = (double *)calloc(n, sizeof(double));
+    s->f = (oreal *)calloc(n * Q, sizeof(oreal));
+    s->ft = (oreal *)calloc(n * Q, sizeof(oreal));
+    s->ua = (oreal *)calloc(n, sizeof(oreal));
+    s->va = (oreal *)calloc(n, sizeof(oreal));
+    s->wa = (oreal *)calloc(n, sizeof(oreal));
+    s->p = (oreal *)calloc(n, sizeof(oreal));
     s->usr = (int *)calloc(n, sizeof(int));
     s->ran = (int *)calloc(n, sizeof(int));
     const int g = s->g;
@@ -252,13 +252,13 @@ orc_ref *orc_ref_create(const orc_cfg *c, const int *ran_in)
                 s->va[L] = 0.0;
                 s->wa[L] = 0.0;
                 if (D == 3)
-                    s->ua[L] = ref_profile(s->u0, z, s->glz) * (double)usr * (1.0 + (double)r * 0.001);
+                    s->ua[L] = (oreal)(ref_profile(s->u0, z, s->glz) * (double)usr * (1.0 + (double)r * 0.001));
                 else if (s->variant == 1)   /* Kokkos/lbm.cpp:123 */
-                    s->ua[L] = ref_profile(s->u0, y, s->gly) * (double)usr * (1.0 + (double)r * 0.001);
+                    s->ua[L] = (oreal)(ref_profile(s->u0, y, s->gly) * (double)usr * (1.0 + (double)r * 0.001));
                 else
-                    s->ua[L] = ref_profile(s->u0, y, s->gly) * (double)usr * (0.9 + (double)r * 0.002);
+                    s->ua[L] = (oreal)(ref_profile(s->u0, y, s->gly) * (double)usr * (0.9 + (double)r * 0.002));
                 for (int ii = 0; ii < Q; ii++) {
-                    RF(s, ii, i, j, k) = feq_ref(&s->L, ii, s->p[L], s->ua[L], s->va[L], s->wa[L]);
+                    RF(s, ii, i, j, k) = (oreal)feq_ref(&s->L, ii, s->p[L], s->ua[L], s->va[L], s->wa[L]);
                     RFT(s, ii, i, j, k) = 0.0;
                 }
             }
@@ -291,8 +291,8 @@ void orc_ref_collision(orc_ref *s)
             for (int i = g; i < s->lx - g; i++) {
                 long L = RN(s, i, j, k);
                 for (int ii = 0; ii < Q; ii++) {
-                    double feq = feq_ref(&s->L, ii, s->p[L], s->ua[L], s->va[L], s->wa[L]);
-                    RF(s, ii, i, j, k) = RF(s, ii, i, j, k) - (RF(s, ii, i, j, k) - feq) / (s->tau0 + 0.5);
+                    oreal feq = feq_p(&s->L, ii, s->p[L], s->ua[L], s->va[L], s->wa[L]);
+                    RF(s, ii, i, j, k) = RF(s, ii, i, j, k) - (RF(s, ii, i, j, k) - feq) / (oreal)(s->tau0 + 0.5);
                 }
             }
 }
@@ -445,31 +445,31 @@ void orc_ref_update(orc_ref *s)
     const lattice *L = &s->L;
     const int Q = L->Q, g = s->g;
     size_t n = (size_t)s->lx * s->ly * s->lz;
-    memset(s->ua, 0, n * sizeof(double));
-    memset(s->va, 0, n * sizeof(double));
-    memset(s->wa, 0, n * sizeof(double));
-    memset(s->p, 0, n * sizeof(double));
+    memset(s->ua, 0, n * sizeof(oreal));
+    memset(s->va, 0, n * sizeof(oreal));
+    memset(s->wa, 0, n * sizeof(oreal));
+    memset(s->p, 0, n * sizeof(oreal));
     for (int k = KLO(s); k < KHI(s); k++)
         for (int j = g; j < s->ly - g; j++)
             for (int i = g; i < s->lx - g; i++) {
                 long N = RN(s, i, j, k);
                 for (int ii = 0; ii < Q; ii++) {
-                    double fv = RF(s, ii, i, j, k);
-                    s->p[N] = s->p[N] + fv / 3.0;
-                    s->ua[N] = s->ua[N] + fv * (double)L->e[ii][0];
-                    s->va[N] = s->va[N] + fv * (double)L->e[ii][1];
-                    if (L->D == 3) s->wa[N] = s->wa[N] + fv * (double)L->e[ii][2];
+                    oreal fv = RF(s, ii, i, j, k);
+                    s->p[N] = s->p[N] + fv / (oreal)3;
+                    s->ua[N] = s->ua[N] + fv * (oreal)L->e[ii][0];
+                    s->va[N] = s->va[N] + fv * (oreal)L->e[ii][1];
+                    if (L->D == 3) s->wa[N] = s->wa[N] + fv * (oreal)L->e[ii][2];
                 }
-                double us = (double)s->usr[N];
+                oreal us = (oreal)s->usr[N];
                 s->ua[N] = s->ua[N] * us;
                 s->va[N] = s->va[N] * us;
                 s->wa[N] = s->wa[N] * us;
                 s->p[N] = s->p[N] * us;
                 long Nin = RN(s, g, j, k);
                 if (L->D == 3)
-                    s->ua[Nin] = ref_profile(s->u0, k - g, s->glz);
+                    s->ua[Nin] = (oreal)ref_profile(s->u0, k - g, s->glz);
                 else
-                    s->ua[Nin] = ref_profile(s->u0, j - g, s->gly);
+                    s->ua[Nin] = (oreal)ref_profile(s->u0, j - g, s->gly);
                 s->va[Nin] = 0.0;
                 s->wa[Nin] = 0.0;
                 s->p[RN(s, s->lx - g - 1, j, k)] = 0.0;
@@ -549,10 +549,12 @@ void orc_ref_populations(const orc_ref *s, double *out)
     }
 }
 
+#ifndef ORC_FLOAT
 const double *orc_ref_raw(const orc_ref *s) { return s->f; }
+#endif
 void orc_ref_set_raw(orc_ref *s, const double *f)
 {
-    memcpy(s->f, f, (size_t)s->lx * s->ly * s->lz * s->L.Q * sizeof(double));
+    for (size_t k = 0; k < (size_t)s->lx * s->ly * s->lz * s->L.Q; k++) s->f[k] = (oreal)f[k];
 }
 
 int orc_ref_forces(const orc_ref *s, double *out, int max)
@@ -576,7 +578,7 @@ struct orc_plain {
     unsigned char *solid;
     double *s;                  /* solid fraction or NULL */
     double *uin;                /* ny*nz */
-    preal *h, *g;               /* SoA [q][N] */
+    oreal *h, *g;               /* SoA [q][N] */
     double *mp, *mu, *mv, *mw;
     double *rowF;               /* 3 per (y,z) row */
     long step;
@@ -662,8 +664,8 @@ orc_plain *orc_plain_create(const orc_cfg *c, const unsigned char *mask, const d
     const size_t N = s->N;
     s->solid = (unsigned char *)calloc(N, 1);
     s->uin = (double *)calloc((size_t)s->ny * s->nz, sizeof(double));
-    s->h = (preal *)malloc(N * Q * sizeof(preal));
-    s->g = (preal *)malloc(N * Q * sizeof(preal));
+    s->h = (oreal *)malloc(N * Q * sizeof(oreal));
+    s->g = (oreal *)malloc(N * Q * sizeof(oreal));
     s->mp = (double *)calloc(N, sizeof(double));
     s->mu = (double *)calloc(N, sizeof(double));
     s->mv = (double *)calloc(N, sizeof(double));
@@ -685,7 +687,7 @@ orc_plain *orc_plain_create(const orc_cfg *c, const unsigned char *mask, const d
         s->solid[n] = (unsigned char)sol;
         double u = 0.0;
         if (!sol) u = s->uin[y + (size_t)s->ny * z] + c->perturb * (xi ? xi[n] : orc_xi(c->seed, (uint64_t)n));
-        for (int ii = 0; ii < Q; ii++) s->h[ii * N + n] = (preal)(sol ? 0.0 : feq_ref(&s->L, ii, 0.0, u, 0.0, 0.0));
+        for (int ii = 0; ii < Q; ii++) s->h[ii * N + n] = (oreal)(sol ? 0.0 : feq_ref(&s->L, ii, 0.0, u, 0.0, 0.0));
         s->mu[n] = u;
     }
     return s;
@@ -701,9 +703,9 @@ void orc_plain_destroy(orc_plain *s)
 /* Zou-He velocity inlet for the shifted incompressible model: for each unknown i (e_ix>0)
  *   g_i = g_bb(i) + 6 w_i U - 18 w_i (e_iy N_y + e_iz N_z),  N_t = sum_{e_kx=0} g_k e_kt,
  * which reproduces sum_i g_i e_i = (U, 0, 0) exactly (no reference counterpart). */
-static void zou_he(const lattice *L, preal *gv, preal U)
+static void zou_he(const lattice *L, oreal *gv, oreal U)
 {
-    preal Ny = 0.0, Nz = 0.0;
+    oreal Ny = 0.0, Nz = 0.0;
     for (int k = 0; k < L->Q; k++) {
         if (L->e[k][0] != 0) continue;
         if (L->e[k][1] > 0) Ny += gv[k]; else if (L->e[k][1] < 0) Ny -= gv[k];
@@ -711,10 +713,10 @@ static void zou_he(const lattice *L, preal *gv, preal U)
     }
     for (int i = 0; i < L->Q; i++) {
         if (L->e[i][0] <= 0) continue;
-        preal tr = 0.0;
+        oreal tr = 0.0;
         if (L->e[i][1] > 0) tr += Ny; else if (L->e[i][1] < 0) tr -= Ny;
         if (L->e[i][2] > 0) tr += Nz; else if (L->e[i][2] < 0) tr -= Nz;
-        gv[i] = (gv[L->bb[i]] + (preal)(6.0 * L->t[i]) * U) - (preal)(18.0 * L->t[i]) * tr;
+        gv[i] = (gv[L->bb[i]] + (oreal)(6.0 * L->t[i]) * U) - (oreal)(18.0 * L->t[i]) * tr;
     }
 }
 
@@ -723,9 +725,9 @@ static void zou_he(const lattice *L, preal *gv, preal U)
  * populations,  g_i = w_i (S + 3 e_ix U + 4.5 Q_i:P),  Q_i = e_i e_i - delta/3.  The sums run
  * in direction order and every product is written out, so the FP64 GPU path (built, like
  * this file, without FMA contraction) reproduces it bit for bit. */
-static void regularize_inlet(const lattice *L, preal *g, preal U)
+static void regularize_inlet(const lattice *L, oreal *g, oreal U)
 {
-    preal S = 0.0, Mxx = 0.0, Myy = 0.0, Mzz = 0.0, Mxy = 0.0, Mxz = 0.0, Myz = 0.0;
+    oreal S = 0.0, Mxx = 0.0, Myy = 0.0, Mzz = 0.0, Mxy = 0.0, Mxz = 0.0, Myz = 0.0;
     for (int k = 0; k < L->Q; k++) {
         const int ex = L->e[k][0], ey = L->e[k][1], ez = L->e[k][2];
         S = S + g[k];
@@ -736,15 +738,15 @@ static void regularize_inlet(const lattice *L, preal *g, preal U)
         if (ex * ez > 0) Mxz = Mxz + g[k]; else if (ex * ez < 0) Mxz = Mxz - g[k];
         if (ey * ez > 0) Myz = Myz + g[k]; else if (ey * ez < 0) Myz = Myz - g[k];
     }
-    const preal S3 = S * (preal)(1.0 / 3.0);
-    const preal Pxx = Mxx - S3, Pyy = Myy - S3, Pzz = Mzz - S3;
-    preal tr = Pxx + Pyy;
+    const oreal S3 = S * (oreal)(1.0 / 3.0);
+    const oreal Pxx = Mxx - S3, Pyy = Myy - S3, Pzz = Mzz - S3;
+    oreal tr = Pxx + Pyy;
     if (L->D == 3) tr = tr + Pzz;
-    const preal d = tr * (preal)(1.0 / 3.0);
-    const preal U3 = (preal)3 * U;
+    const oreal d = tr * (oreal)(1.0 / 3.0);
+    const oreal U3 = (oreal)3 * U;
     for (int i = 0; i < L->Q; i++) {
         const int ex = L->e[i][0], ey = L->e[i][1], ez = L->e[i][2];
-        preal q = 0.0, o = 0.0;
+        oreal q = 0.0, o = 0.0;
         if (ex != 0) q = q + Pxx;
         if (ey != 0) q = q + Pyy;
         if (ez != 0) q = q + Pzz;
@@ -752,9 +754,9 @@ static void regularize_inlet(const lattice *L, preal *g, preal U)
         if (ex * ey > 0) o = o + Mxy; else if (ex * ey < 0) o = o - Mxy;
         if (ex * ez > 0) o = o + Mxz; else if (ex * ez < 0) o = o - Mxz;
         if (ey * ez > 0) o = o + Myz; else if (ey * ez < 0) o = o - Myz;
-        preal t = S;
+        oreal t = S;
         if (ex > 0) t = S + U3; else if (ex < 0) t = S - U3;
-        g[i] = (preal)L->t[i] * (t + ((preal)4.5 * q + (preal)9 * o));
+        g[i] = (oreal)L->t[i] * (t + ((oreal)4.5 * q + (oreal)9 * o));
     }
 }
 
@@ -776,7 +778,7 @@ void orc_plain_step(orc_plain *s, int nsteps)
                 if (s->solid[a]) continue;
                 for (int i = 0; i < Q; i++) {
                     const int *e = L->e[i];
-                    const preal v = s->h[i * N + a];
+                    const oreal v = s->h[i * N + a];
                     int dx = x + e[0], dy = y + e[1], dz = z + e[2];
                     int in = 1;
                     if (s->xper) dx = wrapi(dx, nx); else if (dx < 0 || dx >= nx) in = 0;
@@ -813,10 +815,10 @@ void orc_plain_step(orc_plain *s, int nsteps)
                     }
                 d = (size_t)nx * r;
                 if (!s->solid[d]) {
-                    preal gv[27];
+                    oreal gv[27];
                     for (int i = 0; i < Q; i++) gv[i] = s->g[i * N + d];
-                    zou_he(L, gv, (preal)s->uin[r]);
-                    regularize_inlet(L, gv, (preal)s->uin[r]);
+                    zou_he(L, gv, (oreal)s->uin[r]);
+                    regularize_inlet(L, gv, (oreal)s->uin[r]);
                     for (int i = 0; i < Q; i++) s->g[i * N + d] = gv[i];
                 }
             }
@@ -832,27 +834,27 @@ void orc_plain_step(orc_plain *s, int nsteps)
                     s->mp[a] = s->mu[a] = s->mv[a] = s->mw[a] = 0.0;
                     continue;
                 }
-                preal gv[27];
-                preal p = 0.0, u = 0.0, v = 0.0, w = 0.0;
+                oreal gv[27];
+                oreal p = 0.0, u = 0.0, v = 0.0, w = 0.0;
                 for (int i = 0; i < Q; i++) {
                     gv[i] = s->g[i * N + a];
-                    p = p + gv[i] / (preal)3;
+                    p = p + gv[i] / (oreal)3;
                     const int *e = L->e[i];
                     if (e[0] > 0) u = u + gv[i]; else if (e[0] < 0) u = u - gv[i];
                     if (e[1] > 0) v = v + gv[i]; else if (e[1] < 0) v = v - gv[i];
                     if (e[2] > 0) w = w + gv[i]; else if (e[2] < 0) w = w - gv[i];
                 }
                 s->mp[a] = p; s->mu[a] = u; s->mv[a] = v; s->mw[a] = w;
-                const preal sf = (preal)(s->s ? s->s[a] : 0.0);
+                const oreal sf = (oreal)(s->s ? s->s[a] : 0.0);
                 if (s->s) {
                     F[0] += 2.0 * sf * u;
                     F[1] += 2.0 * sf * v;
                     F[2] += 2.0 * sf * w;
                 }
                 for (int i = 0; i < Q; i++) {
-                    preal feq = feq_p(L, i, p, u, v, w);
-                    preal c = gv[i] - (gv[i] - feq) / (preal)(s->tau0 + 0.5);
-                    if (s->s) c = ((preal)1 - sf) * c + sf * gv[L->bb[i]];
+                    oreal feq = feq_p(L, i, p, u, v, w);
+                    oreal c = gv[i] - (gv[i] - feq) / (oreal)(s->tau0 + 0.5);
+                    if (s->s) c = ((oreal)1 - sf) * c + sf * gv[L->bb[i]];
                     s->h[i * N + a] = c;
                 }
             }
@@ -881,7 +883,7 @@ void orc_plain_populations(const orc_plain *s, double *out)
 }
 void orc_plain_set_populations(orc_plain *s, const double *in)
 {
-    for (size_t k = 0; k < s->N * s->L.Q; k++) s->h[k] = (preal)in[k];
+    for (size_t k = 0; k < s->N * s->L.Q; k++) s->h[k] = (oreal)in[k];
 }
 void orc_plain_last_g(const orc_plain *s, double *out)
 {
@@ -895,7 +897,7 @@ int orc_plain_forces(const orc_plain *s, double *out, int max)
     return s->nrec;
 }
 
-#ifndef ORC_PLAIN_FLOAT
+#ifndef ORC_FLOAT
 /* ==================================================================================== */
 /* Discrete adjoint of the plain-mode step (FP64).  No reference counterpart.            */
 /*                                                                                      */
@@ -1112,4 +1114,4 @@ double orc_adjoint(const orc_cfg *cfg, const unsigned char *mask, const double *
     orc_plain_destroy(s);
     return J;
 }
-#endif /* ORC_PLAIN_FLOAT */
+#endif /* ORC_FLOAT */

@@ -41,9 +41,9 @@ def build() -> None:
     subprocess.run(["make", "-s", "-B", "-C", HERE, "liboracle.so"], check=True)
 
 
-def f32_plain_lib():
-    """The float build of the plain mode (liboracle_f32.so: populations and step arithmetic in
-    float, no adjoint), for the CPU baseline's FP32 leg.  Same create/step/fields interface."""
+def f32_lib():
+    """The float build (liboracle_f32.so: populations, fields and step arithmetic of both modes
+    in float, no adjoint), for the CPU baseline's FP32 legs.  Same create/step interface."""
     path = os.path.join(HERE, "liboracle_f32.so")
     if not os.path.exists(path):
         subprocess.run(["make", "-s", "-B", "-C", HERE, "liboracle_f32.so"], check=True)
@@ -57,6 +57,11 @@ def f32_plain_lib():
     L.orc_plain_populations.argtypes = [P, P]
     L.orc_plain_forces.restype = C.c_int
     L.orc_plain_forces.argtypes = [P, P, C.c_int]
+    L.orc_ref_create.restype = P
+    L.orc_ref_create.argtypes = [C.POINTER(OrcCfg), P]
+    L.orc_ref_destroy.argtypes = [P]
+    L.orc_ref_step.argtypes = [P, C.c_int]
+    L.orc_ref_populations.argtypes = [P, P]
     L.orc_num_threads.restype = C.c_int
     L.orc_set_threads.argtypes = [C.c_int]
     return L

@@ -472,7 +472,7 @@ def test_float_build_tracks_the_fp64_oracle(orc, Q, nx, ny, nz):
     after 60 steps with walls, inlet, outlet and a body its populations stay within FP32
     round-off of the FP64 oracle's, and differ from them (the arithmetic is float)."""
     import ctypes as C
-    L32 = orc.f32_plain_lib()
+    L32 = orc.f32_lib()
     cfg = orc.make_cfg(Q, nx, ny, nz, tau0=0.05, u0=0.05, body=orc.BODY_ZCYL if Q == 27 else orc.BODY_SPHERE,
                        center=(12.0, ny / 2 - 0.5, 0.0), r2=16.0, perturb=1e-3, force_interval=10)
     ref = orc.PlainState(cfg)
@@ -494,3 +494,27 @@ def test_float_build_tracks_the_fp64_oracle(orc, Q, nx, ny, nz):
     F64 = ref.forces()
     assert np.array_equal(F32[:, 0], F64[:, 0])
     assert np.abs(F32[:, 1] - F64[:, 1]).max() < 2e-5 * np.abs(F64[:, 1]).max()   # observed 2.5e-6
+
+
+@pytest.mark.parametrize("Q,nx,ny,nz", [(9, 60, 24, 1), (27, 24, 16, 12)])
+def test_float_build_reference_mode_tracks_the_fp64_oracle(orc, Q, nx, ny, nz):
+    """The float build's reference mode (the CPU baseline's FP32 reference-structured leg): 40
+    steps with the reference's fills, usrbc and body stay within FP32 round-off of the FP64
+    reference mode, and differ from it."""
+    import ctypes as C
+    L32 = orc.f32_lib()
+    cfg = orc.make_cfg(Q, nx, ny, nz, tau0=0.06, u0=0.05, body=orc.BODY_SPHERE,
+                       center=(nx / 4, ny / 2 - 0.5, nz / 2 - 0.5 if Q == 27 else 0.0), r2=9.0)
+    ran = np.random.default_rng(3).integers(0, 100, size=200000).astype(np.int32)
+    ref = orc.RefState(cfg, ran=ran)
+    ref.step(40)
+    h = L32.orc_ref_create(C.byref(cfg), ran.ctypes.data_as(C.c_void_p))
+    assert h
+    try:
+        L32.orc_ref_step(h, 40)
+        f32 = np.zeros((Q, nz, ny, nx))
+        L32.orc_ref_populations(h, f32.ctypes.data_as(C.c_void_p))
+    finally:
+        L32.orc_ref_destroy(h)
+    e = rel_l2(f32, ref.populations())
+    assert 1e-9 < e < 1e-5, e

@@ -60,7 +60,8 @@ MUTANTS = [
      "RF(s, ii, i, j, k) = RF(s, L->bb[ii], i + 2 * e[0], j + 2 * e[1], k + 2 * e[2]);",
      "RF(s, ii, i, j, k) = RF(s, ii, i + 2 * e[0], j + 2 * e[1], k + 2 * e[2]);", ["body9", "body27"]),
     ("collision relaxes with 1/tau0",
-     "(RF(s, ii, i, j, k) - feq) / (s->tau0 + 0.5);", "(RF(s, ii, i, j, k) - feq) / s->tau0;", ["bulk9", "bulk27"]),
+     "(RF(s, ii, i, j, k) - feq) / (oreal)(s->tau0 + 0.5);", "(RF(s, ii, i, j, k) - feq) / (oreal)s->tau0;",
+     ["bulk9", "bulk27"]),
     ("momentum exchange with the sign of e_x flipped",
      "fx += m * (double)e[0];", "fx -= m * (double)e[0];", ["mem9", "mem27"]),
     ("momentum exchange reads f_ii at the solid node",
@@ -83,7 +84,7 @@ MUTANTS = [
      "RFT(s, ii, i, j, k) = RF(s, ii, i - e[0], j - e[1], k - e[2]);",
      "RFT(s, ii, i, j, k) = RF(s, ii, i + e[0], j + e[1], k + e[2]);", ["bulk9", "bulk27"]),
     ("Update divides by 2 instead of 3",
-     "s->p[N] = s->p[N] + fv / 3.0;", "s->p[N] = s->p[N] + fv / 2.0;", ["bulk9", "bulk27"]),
+     "s->p[N] = s->p[N] + fv / (oreal)3;", "s->p[N] = s->p[N] + fv / (oreal)2;", ["bulk9", "bulk27"]),
 ]
 
 RUNNER = r"""
