@@ -107,7 +107,7 @@ extern "C" int lbm_adjoint(lbm_ctx *c, int n0, int n1, double scale, int K, doub
                 aa.f = fa;
                 aa.alpha = weight(m, n0, n1, scale);
                 aa.alpha_next = weight(m + 1, n0, n1, scale);
-                ATRY(adj_step(Q, c->prec, (char *)store + SZ * t, mun, mu, c->flags, c->dsfrac, c->duin, dJds, dJdu, c->rowclean, aa,
+                ATRY(adj_step(Q, c->prec, (char *)store + SZ * t, mun, mu, c->flags, c->dsfrac, c->duin, dJds, dJdu, c->rowclean, c->adj_all_fast, aa,
                               c->stream));
                 std::swap(mun, mu);
             }

@@ -216,33 +216,35 @@ __device__ __forceinline__ void wmoments(const C *f, C &M0, C *M1, C *M2) {
     static_for<0, 6>([&](auto K) { M2[K] = r[4 + K] * (C)sc; });
 }
 
-// Byte offsets, from a row's first element, of direction i's row segment that the fast path
-// reads: mu_{m+1} in row + (e_y + ny e_z) (the gather along +e_i) and H in row - (e_y + ny e_z)
-// (the pull along -e_i).
+// Byte offsets that the straight-line path adds to its row's first element: direction i's row
+// segment is at i * plane + row[(e_y + 1) + 3 (e_z + 1)] for the gather of mu_{m+1} along +e_i and
+// at i * plane - row[...] for the pull of H along -e_i (H and mu elements have one size).
 struct AdjTab {
-    int64_t mun[27];
-    int64_t h[27];
+    int64_t row[9];
+    int64_t plane;
 };
 
 template <int Q, class C, class T>
 static AdjTab adj_tab(const Geo &g) {
-    using L = Lat<Q>;
+    static_assert(sizeof(C) == sizeof(T), "one element size for H and mu");
     AdjTab t{};
-    for (int i = 0; i < Q; i++) {
-        const int64_t dr = (int64_t)L::ey(i) + (int64_t)g.ny * (int64_t)L::ez(i);
-        t.mun[i] = ((int64_t)i * (int64_t)g.ds + dr * (int64_t)g.rs) * (int64_t)sizeof(C);
-        t.h[i] = ((int64_t)i * (int64_t)g.ds - dr * (int64_t)g.rs) * (int64_t)sizeof(T);
-    }
+    for (int ez = -1; ez <= 1; ez++)
+        for (int ey = -1; ey <= 1; ey++)
+            t.row[(ey + 1) + 3 * (ez + 1)] = ((int64_t)ey + (int64_t)g.ny * ez) * (int64_t)g.rs * (int64_t)sizeof(C);
+    t.plane = (int64_t)g.ds * (int64_t)sizeof(C);
     return t;
 }
 
-// Interior rows (no wall or periodic wrap in y/z) whose nodes are all fluid and have no solid
-// neighbour ("clean", flagged per row on the host) take a straight-line path: every source and
-// destination row of a direction is a scalar of the block, and the x edges (inlet, outlet,
-// periodic x) are handled with in-row clamped indices and selects, so every load of the path is
-// issued unconditionally from an address inside its row.  The arithmetic is the general path's,
-// term for term, so both give the same bits.  Other rows take the general path.
-template <int Q, class S, bool SFRAC>
+// Rows whose nodes are all fluid and have no solid neighbour ("clean", flagged per row on the
+// host) and that are not on a periodic y/z edge take a straight-line path: every source and
+// destination row of a direction is a scalar of the block (at a y/z wall, the node's own row and
+// the opposite direction: half-way bounce-back), and the x edges (inlet, outlet, periodic x) are
+// handled with in-row clamped indices and selects, so every load of the path is issued
+// unconditionally from an address inside its row.  The loads are those of the general path, and
+// the arithmetic is shared, so both give the same bits.  Other rows take the general path;
+// GENERAL = false compiles the straight-line path alone (the host launches it when every row
+// qualifies), with the smaller register budget of that path.
+template <int Q, class S, bool SFRAC, bool GENERAL>
 __global__ void __launch_bounds__(256)
     k_adj_step(const typename S::T *__restrict__ H, const typename S::C *__restrict__ mun,
                typename S::C *__restrict__ mu, const uint8_t *__restrict__ flags, const typename S::C *__restrict__ sfrac,
@@ -258,8 +260,9 @@ __global__ void __launch_bounds__(256)
     if (x >= g.nx) return;
     const uint32_t row = (uint32_t)y + (uint32_t)g.ny * (uint32_t)z;
     const uint32_t node = (uint32_t)x + (uint32_t)g.nxp * row;
-    const bool interior = (unsigned)(y - 1) < (unsigned)(g.ny - 2) && (L::D == 2 || (unsigned)(z - 1) < (unsigned)(g.nz - 2));
-    const bool fast = interior && rowclean[row];   // uniform over the block (a block lies in one row)
+    // uniform over the block (a block lies in one row)
+    const bool ylo = y == 0, yhi = y == g.ny - 1, zlo = L::D == 3 && z == 0, zhi = L::D == 3 && z == g.nz - 1;
+    const bool fast = !GENERAL || (rowclean[row] && !(g.yper && (ylo || yhi)) && !(g.zper && (zlo || zhi)));
     const uint8_t fl = fast ? (uint8_t)FL_FLUID : flags[node];
     if (fl == FL_SOLID) {
         static_for<0, Q>([&](auto I) { mu[pel(g, I, (uint32_t)x, row)] = (C)0; });
@@ -283,41 +286,62 @@ __global__ void __launch_bounds__(256)
         const bool xlive_m = x > 0 || g.xper, xlive_p = x + 1 < g.nx || g.xper;
         const char *const Mrow = (const char *)mun + (size_t)row * g.rs * sizeof(C);
         const char *const Hrow = (const char *)H + (size_t)row * g.rs * sizeof(typename S::T);
+        // walls (a clean row at a periodic edge takes the general path): wm, the gather's source
+        // row y + e_y, z + e_z is outside; wh, the pull's source row y - e_y, z - e_z is outside.
+        // Either reads the node's own row, direction opp(i), at its own x.
+        auto wall_at = [&](int ey, int ez) {
+            return (ey > 0 && yhi) || (ey < 0 && ylo) || (ez > 0 && zhi) || (ez < 0 && zlo);
+        };
         C rl[Q], rx[Q];
         typename S::T rh[Q];
+        bool wm[Q];
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
-            constexpr int ex = L::ex(i);
-            const uint32_t bo = (uint32_t)(ex > 0 ? xp : (ex < 0 ? xm : ux)) * (uint32_t)sizeof(C);
-            rl[i] = *(const C *)(Mrow + tab.mun[i] + bo);
+            constexpr int ex = L::ex(i), o = L::opp(i);
+            wm[i] = wall_at(L::ey(i), L::ez(i));
+            const uint32_t bo = wm[i] ? ux * (uint32_t)sizeof(C)
+                                      : (uint32_t)(ex > 0 ? xp : (ex < 0 ? xm : ux)) * (uint32_t)sizeof(C);
+            const int64_t ro = wm[i] ? (int64_t)o * tab.plane : (int64_t)i * tab.plane + tab.row[(L::ey(i) + 1) + 3 * (L::ez(i) + 1)];
+            rl[i] = *(const C *)(Mrow + ro + bo);
         });
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
-            constexpr int ex = L::ex(i);
-            const uint32_t bo = (uint32_t)(ex > 0 ? xm : (ex < 0 ? xp : ux)) * (uint32_t)sizeof(typename S::T);
-            rh[i] = *(const typename S::T *)(Hrow + tab.h[i] + bo);
+            constexpr int ex = L::ex(i), o = L::opp(i);
+            const bool wh = wall_at(-L::ey(i), -L::ez(i));
+            using T = typename S::T;
+            const uint32_t bo = wh ? ux * (uint32_t)sizeof(T)
+                                   : (uint32_t)(ex > 0 ? xm : (ex < 0 ? xp : ux)) * (uint32_t)sizeof(T);
+            const int64_t ro = wh ? (int64_t)o * tab.plane : (int64_t)i * tab.plane - tab.row[(L::ey(i) + 1) + 3 * (L::ez(i) + 1)];
+            rh[i] = *(const T *)(Hrow + ro + bo);
         });
         static_for<0, Q>([&](auto I) {   // the outlet term's row segment at the lane's own x
             constexpr int i = I;
-            if constexpr (L::ex(i) < 0) rx[i] = *(const C *)(Mrow + tab.mun[i] + ux * (uint32_t)sizeof(C));
+            constexpr int o = L::opp(i);
+            if constexpr (L::ex(i) < 0) {
+                const int64_t ro = wm[i] ? (int64_t)o * tab.plane : (int64_t)i * tab.plane + tab.row[(L::ey(i) + 1) + 3 * (L::ez(i) + 1)];
+                rx[i] = *(const C *)(Mrow + ro + ux * (uint32_t)sizeof(C));
+            }
         });
         // keep the selects below after the last load: scheduled among the loads they make the
         // wave wait for the first loads before it issues the rest
         __builtin_amdgcn_sched_barrier(0);
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
-            constexpr int ex = L::ex(i);
-            if constexpr (ex > 0) lam[i] = xlive_p ? rl[i] : (C)0;
+            constexpr int ex = L::ex(i), o = L::opp(i);
+            // a wall entry has no x edge rule (beyond the outlet it still bounces back), but an
+            // inlet unknown stays zero and the outlet term needs a source row in the domain
+            if constexpr (ex > 0) lam[i] = (wm[i] || xlive_p) ? rl[i] : (C)0;
             if constexpr (ex < 0) {
                 const C l = xlive_m ? rl[i] : (C)0;
-                lam[i] = xhi ? l + rx[i] : l;
+                lam[i] = (xhi && !wm[i]) ? l + rx[i] : l;
             }
             if constexpr (ex == 0) lam[i] = rl[i];
-            const C v = S::template decode<i, L>(rh[i], a.f.sp.off[i]);
+            const bool wh = wall_at(-L::ey(i), -L::ez(i));
+            const C v = S::template decode<i, L>(rh[i], wh ? a.f.sp.off[o] : a.f.sp.off[i]);
             if constexpr (ex > 0) gv[i] = xlo ? (C)0 : v;
             else gv[i] = v;
         });
-    } else {
+    } else if constexpr (GENERAL) {
     // ---- 1. lambda = dJ/dH_m(node) ----
     static_for<0, Q>([&](auto I) {
         constexpr int i = I;
@@ -561,32 +585,37 @@ __global__ void __launch_bounds__(256) k_adj_strip(const double *__restrict__ in
 // ------------------------------------------------------------------------------------
 template <int Q, class S>
 static hipError_t adj_step_qs(const void *H, const void *mun, void *mu, const uint8_t *flags, const void *sfrac,
-                              const double *uin, void *dJds, double *dJdu, const uint8_t *rowclean, const AdjArgs &a,
-                              hipStream_t s) {
+                              const double *uin, void *dJds, double *dJdu, const uint8_t *rowclean, bool all_fast,
+                              const AdjArgs &a, hipStream_t s) {
     using T = typename S::T;
     using C = typename S::C;
     // blocks of one row's lanes (at most 256), so the fast path's test is uniform per block
     const Shape sh = lane_shape(a.f.g, 1);
     const dim3 grid(sh.gx, a.f.g.ny, a.f.g.nz);
     const AdjTab tab = adj_tab<Q, C, T>(a.f.g);
-    if (sfrac)
-        hipLaunchKernelGGL((k_adj_step<Q, S, true>), grid, dim3(sh.bx), 0, s, (const T *)H, (const C *)mun, (C *)mu, flags,
-                           (const C *)sfrac, uin, (double *)dJds, dJdu, rowclean, a, tab);
-    else
-        hipLaunchKernelGGL((k_adj_step<Q, S, false>), grid, dim3(sh.bx), 0, s, (const T *)H, (const C *)mun, (C *)mu, flags,
-                           (const C *)nullptr, uin, (double *)dJds, dJdu, rowclean, a, tab);
+    auto go = [&](auto kern, const C *sf) {
+        hipLaunchKernelGGL(kern, grid, dim3(sh.bx), 0, s, (const T *)H, (const C *)mun, (C *)mu, flags, sf, uin,
+                           (double *)dJds, dJdu, rowclean, a, tab);
+    };
+    if (sfrac) {
+        if (all_fast) go(k_adj_step<Q, S, true, false>, (const C *)sfrac);
+        else go(k_adj_step<Q, S, true, true>, (const C *)sfrac);
+    } else {
+        if (all_fast) go(k_adj_step<Q, S, false, false>, (const C *)nullptr);
+        else go(k_adj_step<Q, S, false, true>, (const C *)nullptr);
+    }
     return hipGetLastError();
 }
 
 hipError_t adj_step(int Q, int prec, const void *H, const void *mun, void *mu, const uint8_t *flags, const void *sfrac,
-                    const double *uin, void *dJds, double *dJdu, const uint8_t *rowclean, const AdjArgs &a,
+                    const double *uin, void *dJds, double *dJdu, const uint8_t *rowclean, bool all_fast, const AdjArgs &a,
                     hipStream_t s) {
     if (prec == LBM_FP64)
-        return Q == 9 ? adj_step_qs<9, SF64>(H, mun, mu, flags, sfrac, uin, dJds, dJdu, rowclean, a, s)
-                      : adj_step_qs<27, SF64>(H, mun, mu, flags, sfrac, uin, dJds, dJdu, rowclean, a, s);
+        return Q == 9 ? adj_step_qs<9, SF64>(H, mun, mu, flags, sfrac, uin, dJds, dJdu, rowclean, all_fast, a, s)
+                      : adj_step_qs<27, SF64>(H, mun, mu, flags, sfrac, uin, dJds, dJdu, rowclean, all_fast, a, s);
     if (prec == LBM_FP32)
-        return Q == 9 ? adj_step_qs<9, SF32>(H, mun, mu, flags, sfrac, uin, dJds, dJdu, rowclean, a, s)
-                      : adj_step_qs<27, SF32>(H, mun, mu, flags, sfrac, uin, dJds, dJdu, rowclean, a, s);
+        return Q == 9 ? adj_step_qs<9, SF32>(H, mun, mu, flags, sfrac, uin, dJds, dJdu, rowclean, all_fast, a, s)
+                      : adj_step_qs<27, SF32>(H, mun, mu, flags, sfrac, uin, dJds, dJdu, rowclean, all_fast, a, s);
     return hipErrorInvalidValue;
 }
 

@@ -159,10 +159,11 @@ struct AdjArgs {
     double alpha;        // window weight of this step (dJ/dFx)
     double alpha_next;   // window weight of the next step
 };
-// rowclean: per row (ny*nz bytes), 1 when no node of the row is solid or next to a solid node
+// rowclean: per row (ny*nz bytes), 1 when no node of the row is solid or next to a solid node;
+// all_fast: every row is clean and none lies on a periodic y/z edge (the straight-line kernel alone)
 hipError_t adj_step(int Q, int prec, const void *H, const void *mu_next, void *mu, const uint8_t *flags,
                     const void *sfrac, const double *uin, void *dJds, double *dJdu_rows, const uint8_t *rowclean,
-                    const AdjArgs &a, hipStream_t s);
+                    bool all_fast, const AdjArgs &a, hipStream_t s);
 hipError_t adj_init(int Q, int prec, const void *mu1, const uint8_t *flags, const double *uin,
                     const double *xi, double perturb, uint64_t seed, double alpha1, double *dJdu_rows,
                     const FusedArgs &a, hipStream_t s);
@@ -193,6 +194,7 @@ struct lbm_ctx {
     void *A = nullptr, *B = nullptr;
     uint8_t *flags = nullptr;
     uint8_t *rowclean = nullptr;    // per row: 1 when every node is fluid with no solid neighbour
+    bool adj_all_fast = false;      // every row clean and off the periodic y/z edges
     void *dsfrac = nullptr;
     double *duin = nullptr;
     double *macros = nullptr;       // 4 planes of FP64

@@ -622,12 +622,19 @@ static int init_plain(lbm_ctx *c) {
     HIP_TRY(c, hipMalloc(&c->block_slot, sizeof(int) * std::max<size_t>(slot.size(), 1)));
     HIP_TRY(c, hipMalloc(&c->partials, sizeof(double) * 3 * (std::max(c->nslots, 1) + force_chunks(c->nslots))));
     HIP_TRY(c, hipMemcpyAsync(c->flags, flags.data(), g.plane, hipMemcpyHostToDevice, c->stream));
-    {   // rows whose nodes are all fluid with no solid neighbour (the adjoint's straight-line rows)
-        std::vector<uint8_t> clean((size_t)g.ny * g.nz, 1);
+    {   // rows whose nodes are all fluid with no solid neighbour (the adjoint's straight-line rows;
+        // LBM_ADJ_GENERAL_PATH=1 sends every row through the general path, for tests)
+        std::vector<uint8_t> clean((size_t)g.ny * g.nz, std::getenv("LBM_ADJ_GENERAL_PATH") ? 0 : 1);
         for (size_t r = 0; r < clean.size(); r++)
             for (int x = 0; x < g.nx && clean[r]; x++) clean[r] = flags[(size_t)x + (size_t)g.nxp * r] == FL_FLUID;
         HIP_TRY(c, hipMalloc(&c->rowclean, clean.size()));
         HIP_TRY(c, hipMemcpy(c->rowclean, clean.data(), clean.size(), hipMemcpyHostToDevice));
+        c->adj_all_fast = true;
+        for (int z = 0; z < g.nz; z++)
+            for (int y = 0; y < g.ny; y++) {
+                const bool pedge = (g.yper && (y == 0 || y == g.ny - 1)) || (Q == 27 && g.zper && (z == 0 || z == g.nz - 1));
+                if (!clean[(size_t)y + (size_t)g.ny * z] || pedge) c->adj_all_fast = false;
+            }
     }
     HIP_TRY(c, hipMemcpyAsync(c->duin, c->uin.data(), sizeof(double) * c->uin.size(), hipMemcpyHostToDevice, c->stream));
     if (!slot.empty())

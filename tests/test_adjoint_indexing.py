@@ -56,8 +56,9 @@ def kernel_indices(Q, nx, ny, nz, per, mask):
     for z in range(nz):
         for y in range(ny):
             row = y + ny * z
-            interior = 0 < y < ny - 1 and (Q == 9 or 0 < z < nz - 1)
-            fast = interior and clean[row]
+            ylo, yhi = y == 0, y == ny - 1
+            zlo, zhi = Q == 27 and z == 0, Q == 27 and z == nz - 1
+            fast = clean[row] and not (yper and (ylo or yhi)) and not (zper and (zlo or zhi))
             for x in range(nx):
                 if not fast and solid[z, y, x]:
                     idx += [pel(i, x, row) for i in range(Q)]          # stores of a solid node
@@ -66,17 +67,21 @@ def kernel_indices(Q, nx, ny, nz, per, mask):
                 for i, (ex, ey, ez) in enumerate(E):
                     o = opp(E, i)
                     if fast:
+                        def wall(a, b):
+                            return (a > 0 and yhi) or (a < 0 and ylo) or (b > 0 and zhi) or (b < 0 and zlo)
+                        wm, wh = wall(ey, ez), wall(-ey, -ez)
                         mrow = (row + ((ey + ny * ez) % U32)) % U32
                         dx = x + ex
                         if ex > 0 and dx == nx: dx = 0 if xper else x
                         if ex < 0 and dx < 0: dx = nx - 1 if xper else x
-                        idx.append(pel(i, dx, mrow))
-                        if ex < 0: idx.append(pel(i, x, mrow))     # the outlet term, loaded by every lane
+                        idx.append(pel(o, x, row) if wm else pel(i, dx, mrow))
+                        if ex < 0:                                  # the outlet term, loaded by every lane
+                            idx.append(pel(o, x, row) if wm else pel(i, x, mrow))
                         hrow = (row - ((ey + ny * ez) % U32)) % U32
                         sx = x - ex
                         if ex > 0 and sx < 0: sx = nx - 1 if xper else 0
                         if ex < 0 and sx == nx: sx = 0 if xper else nx - 1
-                        idx.append(pel(i, sx, hrow))
+                        idx.append(pel(o, x, row) if wh else pel(i, sx, hrow))
                     else:
                         # lambda gather, general path
                         if not (ex < 0 and not xper and x == 0):

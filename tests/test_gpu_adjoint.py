@@ -202,3 +202,49 @@ def test_reduced_adjoint3d_matches_cpu_fp64_adjoint(pkg, orc):
         e = (abs(J - Jo) / abs(Jo), rel_l2(ds, dso), rel_l2(np.einsum("kzy,zy->k", phi, du), np.einsum("kzy,zy->k", phi, duo)))
         print(f"[adjoint] reduced 3D sphere {prec} vs CPU FP64: J {e[0]:.2e} dJ/ds {e[1]:.2e} dJ/da {e[2]:.2e}")
         assert max(e) <= tol
+
+
+@pytest.mark.parametrize("case", ["sfrac3d", "mask3d_walls_periodic", "mask2d"])
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_straight_line_path_equals_general_path(pkg, monkeypatch, case, prec):
+    """The reverse step's straight-line path (clean rows, y/z walls included; alone when every
+    row qualifies) against the general path on every row (LBM_ADJ_GENERAL_PATH=1): J, dJ/ds and
+    dJ/du_in bit for bit, on a solid-fraction case (straight-line kernel alone), a 3D mask with
+    walls and a periodic axis, and a 2D mask."""
+    L = pkg.lbm
+    rng = np.random.default_rng(7)
+    if case == "sfrac3d":
+        shape, Q = (10, 12, 40), 27
+        kw = dict(inlet=L.INLET_PARABOLIC)
+    elif case == "mask3d_walls_periodic":
+        shape, Q = (9, 11, 36), 27
+        kw = dict(inlet=L.INLET_PARABOLIC, periodic=(False, False, True), body=L.BODY_MASK)
+    else:
+        shape, Q = (1, 20, 50), 9
+        kw = dict(body=L.BODY_MASK)
+    nz, ny, nx = shape
+    s = np.clip(0.3 * rng.random(shape) - 0.05, 0.0, 1.0)
+    mask = None
+    if case != "sfrac3d":
+        mask = np.zeros(shape, np.uint8)
+        mask[nz // 2, ny // 2, nx // 3] = 1
+        mask[0, 0, nx // 2] = 1            # on a wall row
+    g = L.make_config(Q, nx, ny, nz, tau0=0.1, u0=0.05, precision=L.PRECISIONS[prec], perturb=1e-3, **kw)
+    out = []
+    for general in ("0", "1"):
+        if general == "1":
+            monkeypatch.setenv("LBM_ADJ_GENERAL_PATH", "1")
+        else:
+            monkeypatch.delenv("LBM_ADJ_GENERAL_PATH", raising=False)
+        print(f"[adjoint] straight-line vs general path: {case} {prec} general={general}", flush=True)
+        with pkg.Solver(g) as sv:
+            if mask is not None:
+                sv.set_mask(mask)
+            sv.set_solid_fraction(s)
+            sv.initialize()
+            J, ds, du, _, _ = sv.adjoint(5, 30, 1.0, 7)
+        out.append((J, ds, du))
+    (Ja, dsa, dua), (Jb, dsb, dub) = out
+    assert Ja == Jb
+    assert np.array_equal(dsa, dsb)
+    assert np.array_equal(dua, dub)
