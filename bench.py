@@ -209,11 +209,15 @@ def cpu_reference_structured(seconds=10.0):
                      "sample": f"the reference mode built in float, {n32} steps of the same slab ({dt32:.1f} s)"}}
 
 
-def adjoint_ratio(pkg):
+def adjoint_ratio(pkg, steps=None):
     """Adjoint (recompute + reverse sweep) over forward time on the D3Q27 adjoint workload:
-    192x96x96, FP32, J = mean Cd over steps 2000-6000, 6000 steps, checkpoint every 250."""
+    192x96x96, FP32, J = mean Cd over steps 2000-6000, 6000 steps, checkpoint every 250
+    (`steps`: a shorter window of the same workload, J over its last two thirds, for profiler
+    runs; the per-launch kernels are the same)."""
     W = pkg.configs.adjoint3d_192x96x96()
     n0, n1 = W.window
+    if steps:
+        n0, n1 = max(1, steps // 3), steps
     s = pkg.Solver(W.cfg)
     s.set_solid_fraction(W.extra["sfrac"])
     s.initialize()
@@ -428,6 +432,8 @@ def main():
                     help="instead of the benchmark, time the reference's default 3D and 2D configurations")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="instead of the benchmark, time the adjoint leg alone (forward and reverse sweep)")
+    ap.add_argument("--adjoint-steps", type=int, default=None, metavar="N",
+                    help="with --adjoint-only: a window of N steps (J over the last two thirds) instead of 6000")
     ap.add_argument("--configs-only", action="store_true",
                     help="instead of the benchmark, run the config 0 / config 1 legs alone")
     ap.add_argument("--no-configs", action="store_true")
@@ -446,7 +452,7 @@ def main():
         print(json.dumps({"default_configs": default_configs(pkg)}), flush=True)
         return
     if a.adjoint_only:
-        print(json.dumps({"adjoint": adjoint_ratio(pkg)}), flush=True)
+        print(json.dumps({"adjoint": adjoint_ratio(pkg, a.adjoint_steps)}), flush=True)
         return
     if a.configs_only:
         print(json.dumps({"configs": config_legs(pkg), "dropin_loop": dropin_loop(pkg)}), flush=True)

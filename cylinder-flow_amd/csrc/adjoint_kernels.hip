@@ -303,6 +303,9 @@ __global__ void __launch_bounds__(256)
                                       : (uint32_t)(ex > 0 ? xp : (ex < 0 ? xm : ux)) * (uint32_t)sizeof(C);
             const int64_t ro = wm[i] ? (int64_t)o * tab.plane : (int64_t)i * tab.plane + tab.row[(L::ey(i) + 1) + 3 * (L::ez(i) + 1)];
             rl[i] = *(const C *)(Mrow + ro + bo);
+            // the outlet term's value, from the same row segment at the lane's own x (issued next
+            // to the main load: later, the segment's lines are gone from L2 and fetched again)
+            if constexpr (ex < 0) rx[i] = *(const C *)(Mrow + ro + ux * (uint32_t)sizeof(C));
         });
         static_for<0, Q>([&](auto I) {
             constexpr int i = I;
@@ -313,14 +316,6 @@ __global__ void __launch_bounds__(256)
                                    : (uint32_t)(ex > 0 ? xm : (ex < 0 ? xp : ux)) * (uint32_t)sizeof(T);
             const int64_t ro = wh ? (int64_t)o * tab.plane : (int64_t)i * tab.plane - tab.row[(L::ey(i) + 1) + 3 * (L::ez(i) + 1)];
             rh[i] = *(const T *)(Hrow + ro + bo);
-        });
-        static_for<0, Q>([&](auto I) {   // the outlet term's row segment at the lane's own x
-            constexpr int i = I;
-            constexpr int o = L::opp(i);
-            if constexpr (L::ex(i) < 0) {
-                const int64_t ro = wm[i] ? (int64_t)o * tab.plane : (int64_t)i * tab.plane + tab.row[(L::ey(i) + 1) + 3 * (L::ez(i) + 1)];
-                rx[i] = *(const C *)(Mrow + ro + ux * (uint32_t)sizeof(C));
-            }
         });
         // keep the selects below after the last load: scheduled among the loads they make the
         // wave wait for the first loads before it issues the rest
