@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(256)
     using L = Lat<Q>;
     using C = typename S::C;
     const Geo &g = a.f.g;
-    // the plane z = nz - 1 first: its rows take the general path (at a wall or periodic), whose
+    // the plane z = nz - 1 first: at a periodic z edge its rows take the general path, whose
     // dependent loads make its blocks the slowest; dispatched last they would form the tail
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     const int z = blockIdx.z == 0 ? g.nz - 1 : (int)blockIdx.z - 1;
