@@ -4,16 +4,17 @@
 // so a step costs what a dependent launch costs (~3.4 us on one MI355X; a captured graph of
 // launches costs the same).  Temporal blocking removes all but one launch per K steps: block b
 // owns a W x W core of nodes and loads its core plus a halo of K nodes on each side (T x T
-// nodes, one per thread, T = 24 or 32: the smaller tile when its blocks still fit the CUs
-// once) into registers; each of the K steps writes the
-// block's populations to LDS, pulls from there with the plain-mode boundary rules, and collides
-// in registers.  A halo node's result is exact as long as every node it pulled from was, so the
-// exact region shrinks by one node per step and after K steps covers the core, which is written
-// out.  No block reads another block's output within a launch: there is no grid-wide
-// synchronisation.  Each node's arithmetic is that of k_fused (same pull classification, inlet,
-// moments in the reference's order, feq_ref, relaxation), so the populations equal the
-// one-launch-per-step path bit for bit; forces are summed per tile (a different summation order
-// than the per-row blocks of k_fused, the same value to rounding).
+// nodes, T = 24..32, the smallest side whose blocks still fit the CUs once) into the LDS,
+// coalesced; each of the K steps pulls from the LDS with the plain-mode boundary rules,
+// collides in registers (one thread per node) and writes the block's populations back to the
+// LDS.  A halo node's result is exact as long as every node it pulled from was, so the exact
+// region shrinks by one node per step and after K steps covers the core, which is written out;
+// each step computes only that region (threads in ring order, the waves outside it idle).  No
+// block reads another block's output within a launch: there is no grid-wide synchronisation.
+// Each node's arithmetic is that of k_fused (same pull classification, inlet, moments in the
+// reference's order, feq_ref, relaxation), so the populations equal the one-launch-per-step
+// path bit for bit; forces are summed per tile (a different summation order than the per-row
+// blocks of k_fused, the same value to rounding).
 #include "fused_common.hpp"
 
 #include <cstdlib>
@@ -26,20 +27,85 @@ __global__ void __launch_bounds__(1024)
            const double *__restrict__ uin, double *__restrict__ partials, const FusedArgs a, int K, int steps,
            long step0, int fi, int tiles_x, int ntiles, int pend) {
     using L = Lat<9>;
-    constexpr int Q = 9;
+    // one thread per node of the rings a step can compute (within K - 1 of the core); the
+    // outermost ring is only read, and is loaded with the rest of the tile
+    constexpr int Q = 9, NT = (TB_T - 2) * (TB_T - 2);
     const Geo &g = a.g;
     const int W = TB_T - 2 * K;          // core side
-    extern __shared__ double lds[];      // 2 x Q x TB_T*TB_T: step s writes and reads half s % 2
-    // y fastest: the tile's inlet column (x = 0) lies in one or two waves, so only those run the
-    // inlet's long FP64 sequence (row-major order would put one inlet lane in every wave)
-    const int t = threadIdx.x, ty = t % TB_T, tx = t / TB_T;
-    const int X = (int)(blockIdx.x % tiles_x) * W - K + tx, Y = (int)(blockIdx.x / tiles_x) * W - K + ty;
+    // LDS: two halves of Q x TB_T x TP doubles (step s reads half s % 2; node (tx, ty) at
+    // tx * TP + ty, rows padded to TP = TB_T + 1 so that lanes TP apart hit different banks),
+    // then the tile's flags (TB_T x TP bytes)
+    constexpr int TP = TB_T + 1, HALF = Q * TB_T * TP;
+    extern __shared__ double lds[];
+    uint8_t *lfl = (uint8_t *)(lds + 2 * HALF);
+    // Threads take the tile's nodes ring by ring from the core outwards: threads [0, W*W) the
+    // core (y fastest), then ring r = 1..K-1 (the nodes at Chebyshev distance r from the core: its
+    // left and right columns, then its top and bottom rows).  The nodes a step must compute, the
+    // core and the rings within the steps still to come, are then the first (W + 2m)^2
+    // threads, and the waves beyond them skip the step: the busiest SIMD of a 24 x 24 tile runs
+    // 2+2+2+1+1 wave-steps in 5 steps instead of 3 per step.
+    const int t = threadIdx.x;
+    int tx, ty;
+    if (t < W * W) {
+        tx = K + t / W;
+        ty = K + t % W;
+    } else {
+        int r = 1, side = W + 2;
+        while (t >= side * side) {       // at most K - 1 rings: blockDim.x is NT
+            r++;
+            side += 2;
+        }
+        const int q = t - (side - 2) * (side - 2), lo = K - r, hi = lo + side - 1;
+        if (q < side) {
+            tx = lo;
+            ty = lo + q;
+        } else if (q < 2 * side) {
+            tx = hi;
+            ty = lo + q - side;
+        } else if (q < 3 * side - 2) {
+            ty = lo;
+            tx = lo + 1 + q - 2 * side;
+        } else {
+            ty = hi;
+            tx = lo + 1 + q - (3 * side - 2);
+        }
+    }
+    const int slot = tx * TP + ty;       // LDS index of the node
+    const int X0 = (int)(blockIdx.x % tiles_x) * W - K, Y0 = (int)(blockIdx.x / tiles_x) * W - K;
+    const int X = X0 + tx, Y = Y0 + ty;
     const bool act = X >= 0 && X < g.nx && Y >= 0 && Y < g.ny;
     const bool own = act && tx >= K && tx < K + W && ty >= K && ty < K + W;
-    const uint32_t node = act ? (uint32_t)X + (uint32_t)g.nxp * (uint32_t)Y : 0u;
-    const uint8_t fl = act ? flags[node] : (uint8_t)FL_SOLID;
+    const double uy = uin[act ? Y : 0];
+    // the tile's flags and populations, x fastest (coalesced), into the LDS (populations into
+    // half 0, which the first step reads): node t, t + NT, ... of the tile's T*T; every load is
+    // issued (from node (0, 0) for positions outside the domain) before the first is used
+    {
+        constexpr int NJ = (TB_T * TB_T + NT - 1) / NT;
+        double v[NJ][Q];
+        uint8_t f[NJ];
+#pragma unroll
+        for (int k = 0; k < NJ; k++) {
+            const int j = t + k * NT, lx = j % TB_T, ly = j / TB_T, AX = X0 + lx, AY = Y0 + ly;
+            const bool in = j < TB_T * TB_T && AX >= 0 && AX < g.nx && AY >= 0 && AY < g.ny;
+            const uint32_t ax = in ? (uint32_t)AX : 0u, ay = in ? (uint32_t)AY : 0u;
+            f[k] = flags[ax + (uint32_t)g.nxp * ay];
+            static_for<0, Q>([&](auto I) { v[k][I] = A[pel(g, I, ax, ay)]; });
+        }
+#pragma unroll
+        for (int k = 0; k < NJ; k++) {
+            const int j = t + k * NT, lx = j % TB_T, ly = j / TB_T, AX = X0 + lx, AY = Y0 + ly;
+            const bool in = AX >= 0 && AX < g.nx && AY >= 0 && AY < g.ny;
+            if (j < TB_T * TB_T) {
+                lfl[lx * TP + ly] = in ? f[k] : (uint8_t)FL_SOLID;
+                static_for<0, Q>([&](auto I) { lds[I * TB_T * TP + lx * TP + ly] = in ? v[k][I] : 0.0; });
+            }
+        }
+    }
+    __syncthreads();
+    const uint8_t fl = lfl[slot];
     const bool live = fl != FL_SOLID;
-    // sources that are solid (mask bounce-back), classified once as the pull does
+    // sources that are solid (mask bounce-back), classified once as the pull does (the sources
+    // of rings within K - 1 lie in the tile)
     uint32_t solid_src = 0;
     if (fl == FL_NEAR) {
         static_for<1, Q>([&](auto I) {
@@ -50,62 +116,68 @@ __global__ void __launch_bounds__(1024)
             if ((unsigned)sy >= (unsigned)g.ny) return;          // wall
             int sx = X - ex;
             if (ex < 0 && X == g.nx - 1) sx = g.nx - 1;          // outlet clamp
-            if (flags[(uint32_t)sx + (uint32_t)g.nxp * (uint32_t)sy] == FL_SOLID) solid_src |= 1u << i;
+            if (lfl[(tx + sx - X) * TP + ty - ey] == FL_SOLID) solid_src |= 1u << i;
         });
     }
     double h[Q];
-    static_for<0, Q>([&](auto I) { h[I] = act ? A[pel(g, I, (uint32_t)(act ? X : 0), (uint32_t)(act ? Y : 0))] : 0.0; });
-    const double uy = act ? uin[Y] : 0.0;
+    static_for<0, Q>([&](auto I) { h[I] = lds[I * TB_T * TP + slot]; });
     for (int s = 0; s < steps; s++) {
         const long step = step0 + s + 1;
         const bool force = ANY_FORCE && fi > 0 && step % fi == 0;
         // two halves: the barrier of step s also orders the reads of step s - 1 before the writes
         // of step s + 1 into the same half
-        double *buf = lds + (size_t)(s & 1) * Q * TB_T * TB_T;
-        static_for<0, Q>([&](auto I) { buf[I * TB_T * TB_T + t] = h[I]; });
-        __syncthreads();
-        double gv[Q];
+        double *buf = lds + (size_t)(s & 1) * HALF;
+        // this step computes the rings within rs = steps - 1 - s of the core, from the values of
+        // the rings within rs + 1 (computed by the previous step, or loaded into half 0)
+        const int rs = steps - 1 - s, ncomp = (W + 2 * rs) * (W + 2 * rs), nsrc = (W + 2 * rs + 2) * (W + 2 * rs + 2);
+        if (s > 0) {
+            if (t < nsrc) static_for<0, Q>([&](auto I) { buf[I * TB_T * TP + slot] = h[I]; });
+            __syncthreads();
+        }
         double fx = 0.0, fy = 0.0;
-        static_for<0, Q>([&](auto I) {
-            constexpr int i = I;
-            constexpr int ex = L::ex(i), ey = L::ey(i), o = L::opp(i);
-            const bool zh = ex > 0 && X == 0;
-            const int sy = Y - ey;
-            const bool wall = !zh && (unsigned)sy >= (unsigned)g.ny;
-            const bool solid = (solid_src >> i) & 1u;
-            int sx = X - ex;
-            if (ex < 0 && X == g.nx - 1) sx = g.nx - 1;
-            // tile coordinates of the source, kept inside the tile (a source beyond the tile only
-            // feeds halo nodes whose results no core node uses within the launch)
-            int stx = tx + (sx - X), sty = ty - ey;
-            stx = stx < 0 ? 0 : (stx >= TB_T ? TB_T - 1 : stx);
-            sty = sty < 0 ? 0 : (sty >= TB_T ? TB_T - 1 : sty);
-            const double pulled = buf[i * TB_T * TB_T + stx * TB_T + sty];
-            const double v = (wall || solid) ? h[o] : pulled;
-            gv[i] = (live && !zh) ? v : 0.0;
-            if (ANY_FORCE && force && solid) {
-                const double m = 2.0 * gv[i];
-                if constexpr (ex != 0) fx -= ex * m;
-                if constexpr (ey != 0) fy -= ey * m;
-            }
-        });
-        if (X == 0 && live) velocity_inlet<Q, double>(gv, uy);
-        // moments in the reference's term order (lbm.cpp:390-397), relaxation f - (f - feq)/tau
-        double p = 0.0, u = 0.0, v = 0.0, w = 0.0;
-        static_for<0, Q>([&](auto I) {
-            constexpr int i = I;
-            p = p + div_cr(gv[i], 3.0, 1.0 / 3.0);
-            if constexpr (L::ex(i) > 0) u = u + gv[i];
-            else if constexpr (L::ex(i) < 0) u = u - gv[i];
-            if constexpr (L::ey(i) > 0) v = v + gv[i];
-            else if constexpr (L::ey(i) < 0) v = v - gv[i];
-        });
-        static_for<0, Q>([&](auto I) {
-            constexpr int i = I;
-            const double feq = feq_ref<Q, i>(p, u, v, w);
-            const double c = gv[i] - div_cr(gv[i] - feq, a.tau, a.rtau);
-            h[i] = live ? c : 0.0;
-        });
+        if (t < ncomp) {
+            double gv[Q];
+            static_for<0, Q>([&](auto I) {
+                constexpr int i = I;
+                constexpr int ex = L::ex(i), ey = L::ey(i), o = L::opp(i);
+                const bool zh = ex > 0 && X == 0;
+                const int sy = Y - ey;
+                const bool wall = !zh && (unsigned)sy >= (unsigned)g.ny;
+                const bool solid = (solid_src >> i) & 1u;
+                int sx = X - ex;
+                if (ex < 0 && X == g.nx - 1) sx = g.nx - 1;
+                // tile coordinates of the source (within ring rs + 1, so inside the tile; the clamp
+                // only keeps every LDS index in range)
+                int stx = tx + (sx - X), sty = ty - ey;
+                stx = stx < 0 ? 0 : (stx >= TB_T ? TB_T - 1 : stx);
+                sty = sty < 0 ? 0 : (sty >= TB_T ? TB_T - 1 : sty);
+                const double pulled = buf[i * TB_T * TP + stx * TP + sty];
+                const double v = (wall || solid) ? h[o] : pulled;
+                gv[i] = (live && !zh) ? v : 0.0;
+                if (ANY_FORCE && force && solid) {
+                    const double m = 2.0 * gv[i];
+                    if constexpr (ex != 0) fx -= ex * m;
+                    if constexpr (ey != 0) fy -= ey * m;
+                }
+            });
+            if (X == 0 && live) velocity_inlet<Q, double>(gv, uy);
+            // moments in the reference's term order (lbm.cpp:390-397), relaxation f - (f - feq)/tau
+            double p = 0.0, u = 0.0, v = 0.0, w = 0.0;
+            static_for<0, Q>([&](auto I) {
+                constexpr int i = I;
+                p = p + div_cr(gv[i], 3.0, 1.0 / 3.0);
+                if constexpr (L::ex(i) > 0) u = u + gv[i];
+                else if constexpr (L::ex(i) < 0) u = u - gv[i];
+                if constexpr (L::ey(i) > 0) v = v + gv[i];
+                else if constexpr (L::ey(i) < 0) v = v - gv[i];
+            });
+            static_for<0, Q>([&](auto I) {
+                constexpr int i = I;
+                const double feq = feq_ref<Q, i>(p, u, v, w);
+                const double c = gv[i] - div_cr(gv[i] - feq, a.tau, a.rtau);
+                h[i] = live ? c : 0.0;
+            });
+        }
         if (ANY_FORCE && force) {   // uniform over the grid
             double fz = 0.0;
             if (!own) fx = fy = 0.0;
@@ -121,7 +193,18 @@ __global__ void __launch_bounds__(1024)
             }
         }
     }
-    if (own && live) static_for<0, Q>([&](auto I) { __builtin_nontemporal_store(h[I], &B[pel(g, I, (uint32_t)X, (uint32_t)Y)]); });
+    // the core through the LDS half the last step did not read (its last reader, step
+    // steps - 2, finished before the barrier of step steps - 1), then stored x fastest
+    double *out = lds + (size_t)(steps & 1) * HALF;
+    if (t < W * W) static_for<0, Q>([&](auto I) { out[I * TB_T * TP + slot] = h[I]; });
+    __syncthreads();
+    for (int j = t; j < W * W; j += NT) {
+        const int lx = K + j % W, ly = K + j / W, AX = X0 + lx, AY = Y0 + ly;
+        if (AX < g.nx && AY < g.ny && lfl[lx * TP + ly] != FL_SOLID)
+            static_for<0, Q>([&](auto I) {
+                __builtin_nontemporal_store(out[I * TB_T * TP + lx * TP + ly], &B[pel(g, I, (uint32_t)AX, (uint32_t)AY)]);
+            });
+    }
 }
 
 // The force records of one launch: block r sums the ntiles partials of record r in a fixed order
@@ -156,7 +239,8 @@ int tb_depth() {
     return 5;
 }
 
-size_t tb_lds_bytes(int T) { return sizeof(double) * 2 * 9 * T * T; }
+// two population halves of 9 x T x (T + 1) doubles and the tile's flags (T x (T + 1) bytes)
+size_t tb_lds_bytes(int T) { return sizeof(double) * 2 * 9 * T * (T + 1) + (size_t)T * (T + 1); }
 
 int tb_tiles(const Geo &g, int T, int K, int *tiles_x) {
     const int W = T - 2 * K;
@@ -164,29 +248,39 @@ int tb_tiles(const Geo &g, int T, int K, int *tiles_x) {
     return *tiles_x * ((g.ny + W - 1) / W);
 }
 
-// the tile side for a domain: 24 when its tiles fit the CUs once (and leave a core of at least
-// 2 x 2 nodes), else 32; LBM_TB_TILE=24 or 32 overrides where the core stays at least 2 x 2
+// the tile sides instantiated, and the tile side for a domain: the smallest whose tiles fit the
+// CUs once and leave a core of at least 2 x 2 nodes (deeper launches need wider halos), else
+// the largest; LBM_TB_TILE overrides with any instantiated side that leaves a 2 x 2 core
+static const int kTbSides[] = {24, 26, 28, 30, 32};
+
 int tb_tile(const Geo &g, int K, int cus) {
     if (const char *e = std::getenv("LBM_TB_TILE")) {
         const int t = std::atoi(e);
-        if ((t == 24 || t == 32) && t - 2 * K >= 2) return t;
+        for (int T : kTbSides)
+            if (t == T && t - 2 * K >= 2) return t;
     }
     int tx = 0;
-    return 24 - 2 * K >= 2 && tb_tiles(g, 24, K, &tx) <= cus ? 24 : 32;
+    for (int T : kTbSides)
+        if (T - 2 * K >= 2 && tb_tiles(g, T, K, &tx) <= cus) return T;
+    return 32;
+}
+
+template <int T>
+static hipError_t tb_attr() {
+    hipError_t e = hipFuncSetAttribute((const void *)k_tb2d<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)tb_lds_bytes(T));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_tb2d<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)tb_lds_bytes(T));
+    return e;
 }
 
 hipError_t tb_prepare() {
-    hipError_t e = hipFuncSetAttribute((const void *)k_tb2d<24, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)tb_lds_bytes(24));
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_tb2d<24, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)tb_lds_bytes(24));
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_tb2d<32, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)tb_lds_bytes(32));
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_tb2d<32, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)tb_lds_bytes(32));
+    hipError_t e = tb_attr<24>();
+    if (e == hipSuccess) e = tb_attr<26>();
+    if (e == hipSuccess) e = tb_attr<28>();
+    if (e == hipSuccess) e = tb_attr<30>();
+    if (e == hipSuccess) e = tb_attr<32>();
     return e;
 }
 
@@ -205,16 +299,18 @@ hipError_t tb_steps(const void *A, void *B, const uint8_t *flags, const double *
     const int nforce = fi > 0 ? (int)((step0 + steps) / fi - step0 / fi) : 0;
     *nrec = nforce;
     if (pend < 0 || pend + nforce > tb_records(K)) return hipErrorInvalidValue;
-    auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(ntiles), dim3(T * T), lds, s, (const double *)A, (double *)B, flags, uin, partials, a,
-                           K, steps, step0, fi, tiles_x, ntiles, pend);
+    if (T - 2 * K < 1) return hipErrorInvalidValue;
+    auto go = [&](auto kern) {   // (T - 2)^2 threads: the kernel's thread map assumes exactly that
+        hipLaunchKernelGGL(kern, dim3(ntiles), dim3((T - 2) * (T - 2)), lds, s, (const double *)A, (double *)B, flags, uin,
+                           partials, a, K, steps, step0, fi, tiles_x, ntiles, pend);
     };
-    if (T == 24) {
-        if (nforce > 0) go(k_tb2d<24, true>);
-        else go(k_tb2d<24, false>);
-    } else {
-        if (nforce > 0) go(k_tb2d<32, true>);
-        else go(k_tb2d<32, false>);
+    switch (T) {
+    case 24: nforce > 0 ? go(k_tb2d<24, true>) : go(k_tb2d<24, false>); break;
+    case 26: nforce > 0 ? go(k_tb2d<26, true>) : go(k_tb2d<26, false>); break;
+    case 28: nforce > 0 ? go(k_tb2d<28, true>) : go(k_tb2d<28, false>); break;
+    case 30: nforce > 0 ? go(k_tb2d<30, true>) : go(k_tb2d<30, false>); break;
+    case 32: nforce > 0 ? go(k_tb2d<32, true>) : go(k_tb2d<32, false>); break;
+    default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

@@ -4,6 +4,8 @@ FP64 runs are compared at round-off level (the kernels follow the oracle's round
 both built without FMA contraction); forces differ only by summation order.  FP32 and the
 16-bit storage formats are compared to the FP64 oracle with stated error budgets.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -499,6 +501,21 @@ def test_temporal_blocking_equals_one_launch_per_step(pkg, monkeypatch, fi, case
     disk and on a mask that touches the inlet and outlet columns and the walls; with a record
     every step, one call of 700 steps holds more records than the per-tile partials' slots, so
     they are summed in several batches within the call."""
+    _tb_vs_one_launch(pkg, monkeypatch, fi, case)
+
+
+@pytest.mark.parametrize("tile,depth", [("32", "5"), ("24", "3"), ("32", "8"), ("24", "1")])
+def test_temporal_blocking_tile_and_depth_variants(pkg, monkeypatch, tile, depth):
+    """The same comparison for the other tile sides and depths the host can select (the 32 x 32
+    tile is the default where 24 x 24 tiles would not fit the CUs once): each step computes
+    only the rings of the tile that later steps read, so every (tile, depth) pair changes which
+    waves run which step."""
+    monkeypatch.setenv("LBM_TB_TILE", tile)
+    monkeypatch.setenv("LBM_TB_DEPTH", depth)
+    _tb_vs_one_launch(pkg, monkeypatch, 3, "edges")
+
+
+def _tb_vs_one_launch(pkg, monkeypatch, fi, case):
     L = pkg.lbm
     nx, ny = 100, 40
     kw = dict(tau0=0.06, u0=0.05, precision=L.FP64, bc_mode=L.BC_PLAIN, perturb=1e-3, force_interval=fi)
@@ -518,7 +535,8 @@ def test_temporal_blocking_equals_one_launch_per_step(pkg, monkeypatch, fi, case
     out = []
     for tb in ("0", "1"):
         monkeypatch.setenv("LBM_TEMPORAL_BLOCKING", tb)
-        print(f"[parity] temporal blocking {tb} case={case} fi={fi}", flush=True)
+        print(f"[parity] temporal blocking {tb} case={case} fi={fi} tile={os.environ.get('LBM_TB_TILE')} "
+              f"depth={os.environ.get('LBM_TB_DEPTH')}", flush=True)
         s = pkg.Solver(g)
         if mask is not None:
             s.set_mask(mask)
