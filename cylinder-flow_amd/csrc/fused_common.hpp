@@ -136,14 +136,19 @@ __device__ __forceinline__ void velocity_inlet(C *g, C U) {
 
 // Block-wide sum of three doubles in a fixed order (wave butterfly, then the waves in order);
 // the result is valid in thread 0.  `red` holds 3 doubles per wave of the block.
-template <int NW>
-__device__ __forceinline__ void block_sum3(double &a, double &b, double &c, double *red) {
+// wave butterfly of three doubles: every lane ends with the wave's sums
+__device__ __forceinline__ void wave_sum3(double &a, double &b, double &c) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         a += __shfl_xor(a, off);
         b += __shfl_xor(b, off);
         c += __shfl_xor(c, off);
     }
+}
+
+template <int NW>
+__device__ __forceinline__ void block_sum3(double &a, double &b, double &c, double *red) {
+    wave_sum3(a, b, c);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (lane == 0) {
         red[3 * wv] = a;

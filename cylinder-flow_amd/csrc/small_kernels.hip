@@ -21,6 +21,8 @@
 
 namespace lbm {
 
+constexpr int TB_KMAX = 10;   // steps per launch at most (tb_depth)
+
 template <int TB_T, bool ANY_FORCE>
 __global__ void __launch_bounds__(1024)
     k_tb2d(const double *__restrict__ A, double *__restrict__ B, const uint8_t *__restrict__ flags,
@@ -30,6 +32,7 @@ __global__ void __launch_bounds__(1024)
     // one thread per node of the rings a step can compute (within K - 1 of the core); the
     // outermost ring is only read, and is loaded with the rest of the tile
     constexpr int Q = 9, NT = (TB_T - 2) * (TB_T - 2);
+    static_assert((NT + 63) / 64 <= 16, "force records hold 16 waves");
     const Geo &g = a.g;
     const int W = TB_T - 2 * K;          // core side
     // LDS: two halves of Q x TB_T x TP doubles (step s reads half s % 2; node (tx, ty) at
@@ -121,9 +124,13 @@ __global__ void __launch_bounds__(1024)
     }
     double h[Q];
     static_for<0, Q>([&](auto I) { h[I] = lds[I * TB_T * TP + slot]; });
+    // force records: the steps s = sf, sf + fi, ... of this launch (step0 + s + 1 a multiple of
+    // fi); each wave's sums go to red[record][wave] and are added, waves in order, after the
+    // launch's last barrier, so no force step waits for a block-wide sum
+    __shared__ double red[3 * 16 * TB_KMAX];
+    int sf = ANY_FORCE && fi > 0 ? (int)(fi - 1 - step0 % fi) : -1, kr = 0;
     for (int s = 0; s < steps; s++) {
-        const long step = step0 + s + 1;
-        const bool force = ANY_FORCE && fi > 0 && step % fi == 0;
+        const bool force = ANY_FORCE && s == sf;
         // two halves: the barrier of step s also orders the reads of step s - 1 before the writes
         // of step s + 1 into the same half
         double *buf = lds + (size_t)(s & 1) * HALF;
@@ -154,12 +161,20 @@ __global__ void __launch_bounds__(1024)
                 const double pulled = buf[i * TB_T * TP + stx * TP + sty];
                 const double v = (wall || solid) ? h[o] : pulled;
                 gv[i] = (live && !zh) ? v : 0.0;
-                if (ANY_FORCE && force && solid) {
-                    const double m = 2.0 * gv[i];
-                    if constexpr (ex != 0) fx -= ex * m;
-                    if constexpr (ey != 0) fy -= ey * m;
-                }
             });
+            // momentum exchange of the mask links, in direction order, from the pulled values
+            // (before the inlet rule), outside the pull so the other steps run without it
+            if (ANY_FORCE && force && solid_src != 0u) {
+                static_for<1, Q>([&](auto I) {
+                    constexpr int i = I;
+                    constexpr int ex = L::ex(i), ey = L::ey(i);
+                    if ((solid_src >> i) & 1u) {
+                        const double m = 2.0 * gv[i];
+                        if constexpr (ex != 0) fx -= ex * m;
+                        if constexpr (ey != 0) fy -= ey * m;
+                    }
+                });
+            }
             if (X == 0 && live) velocity_inlet<Q, double>(gv, uy);
             // moments in the reference's term order (lbm.cpp:390-397), relaxation f - (f - feq)/tau
             double p = 0.0, u = 0.0, v = 0.0, w = 0.0;
@@ -181,16 +196,15 @@ __global__ void __launch_bounds__(1024)
         if (ANY_FORCE && force) {   // uniform over the grid
             double fz = 0.0;
             if (!own) fx = fy = 0.0;
-            __shared__ double red[3 * 16];
-            block_sum3<16>(fx, fy, fz, red);
-            if (t == 0) {
-                // record slot: the records already pending, then the index within the launch
-                const int r = pend + (int)(step / fi - (step0 + 1 + fi - 1) / fi);
-                double *P = partials + 3 * ((size_t)r * ntiles + blockIdx.x);
-                P[0] = fx;
-                P[1] = fy;
-                P[2] = fz;
+            wave_sum3(fx, fy, fz);
+            if ((t & 63) == 0) {
+                double *R = red + 3 * (16 * kr + (t >> 6));
+                R[0] = fx;
+                R[1] = fy;
+                R[2] = fz;
             }
+            kr++;
+            sf += fi;
         }
     }
     // the core through the LDS half the last step did not read (its last reader, step
@@ -198,6 +212,21 @@ __global__ void __launch_bounds__(1024)
     double *out = lds + (size_t)(steps & 1) * HALF;
     if (t < W * W) static_for<0, Q>([&](auto I) { out[I * TB_T * TP + slot] = h[I]; });
     __syncthreads();
+    if (ANY_FORCE && t < kr) {
+        // record t of this launch into slot pend + t: the waves' sums in wave order
+        constexpr int NW = (NT + 63) / 64;
+        const double *R = red + 3 * 16 * t;
+        double fx = R[0], fy = R[1], fz = R[2];
+        for (int k = 1; k < NW; k++) {
+            fx += R[3 * k];
+            fy += R[3 * k + 1];
+            fz += R[3 * k + 2];
+        }
+        double *P = partials + 3 * ((size_t)(pend + t) * ntiles + blockIdx.x);
+        P[0] = fx;
+        P[1] = fy;
+        P[2] = fz;
+    }
     for (int j = t; j < W * W; j += NT) {
         const int lx = K + j % W, ly = K + j / W, AX = X0 + lx, AY = Y0 + ly;
         if (AX < g.nx && AY < g.ny && lfl[lx * TP + ly] != FL_SOLID)
@@ -234,7 +263,7 @@ __global__ void __launch_bounds__(256) k_tb_force(const double *__restrict__ par
 int tb_depth() {
     if (const char *e = std::getenv("LBM_TB_DEPTH")) {
         const int k = std::atoi(e);
-        if (k >= 1 && k <= 10) return k;
+        if (k >= 1 && k <= TB_KMAX) return k;
     }
     return 5;
 }
@@ -299,7 +328,7 @@ hipError_t tb_steps(const void *A, void *B, const uint8_t *flags, const double *
     const int nforce = fi > 0 ? (int)((step0 + steps) / fi - step0 / fi) : 0;
     *nrec = nforce;
     if (pend < 0 || pend + nforce > tb_records(K)) return hipErrorInvalidValue;
-    if (T - 2 * K < 1) return hipErrorInvalidValue;
+    if (T - 2 * K < 1 || K > TB_KMAX || steps < 1 || steps > K) return hipErrorInvalidValue;
     auto go = [&](auto kern) {   // (T - 2)^2 threads: the kernel's thread map assumes exactly that
         hipLaunchKernelGGL(kern, dim3(ntiles), dim3((T - 2) * (T - 2)), lds, s, (const double *)A, (double *)B, flags, uin,
                            partials, a, K, steps, step0, fi, tiles_x, ntiles, pend);
