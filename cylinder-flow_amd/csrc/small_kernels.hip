@@ -101,6 +101,8 @@ __global__ void __launch_bounds__(1024)
             if (j < TB_T * TB_T) {
                 lfl[lx * TP + ly] = in ? f[k] : (uint8_t)FL_SOLID;
                 static_for<0, Q>([&](auto I) { lds[I * TB_T * TP + lx * TP + ly] = in ? v[k][I] : 0.0; });
+                // positions outside the domain read 0 in both halves (the inlet's unknowns)
+                if (!in) static_for<0, Q>([&](auto I) { lds[HALF + I * TB_T * TP + lx * TP + ly] = 0.0; });
             }
         }
     }
@@ -122,6 +124,21 @@ __global__ void __launch_bounds__(1024)
             if (lfl[(tx + sx - X) * TP + ty - ey] == FL_SOLID) solid_src |= 1u << i;
         });
     }
+    // From the second step on, every pull is a plain LDS read at the source's position (no
+    // boundary rule in the pull): each live node also writes, with its own values, the values
+    // its rules pull from positions no live node owns: h_opp(i) into the source position of
+    // each bounce-back link i (a wall or a solid source: only this node pulls direction i from
+    // there), and at the outlet (x = nx - 1) its values of the directions with e_x < 0 into the
+    // position beyond the outlet (x = nx), which the outlet node above or below pulls for the
+    // zero gradient.  The inlet's unknowns read positions outside the domain, which hold 0.
+    uint32_t bb = 0;
+    static_for<1, Q>([&](auto I) {
+        constexpr int i = I;
+        constexpr int ex = L::ex(i), ey = L::ey(i);
+        if (!(ex > 0 && X == 0) && (unsigned)(Y - ey) >= (unsigned)g.ny) bb |= 1u << i;
+    });
+    bb = live ? (bb | solid_src) : 0u;
+    const bool outl = live && X == g.nx - 1;
     double h[Q];
     static_for<0, Q>([&](auto I) { h[I] = lds[I * TB_T * TP + slot]; });
     // force records: the steps s = sf, sf + fi, ... of this launch (step0 + s + 1 a multiple of
@@ -138,12 +155,32 @@ __global__ void __launch_bounds__(1024)
         // the rings within rs + 1 (computed by the previous step, or loaded into half 0)
         const int rs = steps - 1 - s, ncomp = (W + 2 * rs) * (W + 2 * rs), nsrc = (W + 2 * rs + 2) * (W + 2 * rs + 2);
         if (s > 0) {
-            if (t < nsrc) static_for<0, Q>([&](auto I) { buf[I * TB_T * TP + slot] = h[I]; });
+            if (t < nsrc && live) {
+                static_for<0, Q>([&](auto I) { buf[I * TB_T * TP + slot] = h[I]; });
+                if (bb != 0u)
+                    static_for<1, Q>([&](auto I) {
+                        constexpr int i = I;
+                        constexpr int ex = L::ex(i), ey = L::ey(i), o = L::opp(i);
+                        if ((bb >> i) & 1u) buf[i * TB_T * TP + slot - ex * TP - ey] = h[o];
+                    });
+                if (outl)
+                    static_for<1, Q>([&](auto I) {
+                        constexpr int i = I;
+                        if constexpr (L::ex(i) < 0) buf[i * TB_T * TP + slot + TP] = h[i];
+                    });
+            }
             __syncthreads();
         }
         double fx = 0.0, fy = 0.0;
         if (t < ncomp) {
             double gv[Q];
+            if (s > 0) {
+                static_for<0, Q>([&](auto I) {
+                    constexpr int i = I;
+                    gv[i] = buf[i * TB_T * TP + slot - L::ex(i) * TP - L::ey(i)];
+                });
+            } else {
+            // the first step pulls from the loaded tile with the rules
             static_for<0, Q>([&](auto I) {
                 constexpr int i = I;
                 constexpr int ex = L::ex(i), ey = L::ey(i), o = L::opp(i);
@@ -162,6 +199,7 @@ __global__ void __launch_bounds__(1024)
                 const double v = (wall || solid) ? h[o] : pulled;
                 gv[i] = (live && !zh) ? v : 0.0;
             });
+            }
             // momentum exchange of the mask links, in direction order, from the pulled values
             // (before the inlet rule), outside the pull so the other steps run without it
             if (ANY_FORCE && force && solid_src != 0u) {
@@ -189,8 +227,7 @@ __global__ void __launch_bounds__(1024)
             static_for<0, Q>([&](auto I) {
                 constexpr int i = I;
                 const double feq = feq_ref<Q, i>(p, u, v, w);
-                const double c = gv[i] - div_cr(gv[i] - feq, a.tau, a.rtau);
-                h[i] = live ? c : 0.0;
+                h[i] = gv[i] - div_cr(gv[i] - feq, a.tau, a.rtau);   // (a solid node's values are never used)
             });
         }
         if (ANY_FORCE && force) {   // uniform over the grid
@@ -259,13 +296,15 @@ __global__ void __launch_bounds__(256) k_tb_force(const double *__restrict__ par
     }
 }
 
-// steps per launch: 5 (LBM_TB_DEPTH overrides, 1-10)
+// steps per launch: 7 (LBM_TB_DEPTH overrides, 1-10); with the 14 x 14 core of a 400 x 100
+// domain's 28 x 28 tiles, 7 steps per launch measured fastest (2.54 us per step against 2.61 at
+// 5 steps on 24 x 24 tiles and 2.60 at 8 on 30 x 30)
 int tb_depth() {
     if (const char *e = std::getenv("LBM_TB_DEPTH")) {
         const int k = std::atoi(e);
         if (k >= 1 && k <= TB_KMAX) return k;
     }
-    return 5;
+    return 7;
 }
 
 // two population halves of 9 x T x (T + 1) doubles and the tile's flags (T x (T + 1) bytes)

@@ -497,14 +497,14 @@ def test_fields_on_request_leave_the_state_unchanged(pkg, prec, bc, Q):
 def test_temporal_blocking_equals_one_launch_per_step(pkg, monkeypatch, fi, case):
     """Several D2Q9 FP64 steps per launch (small domains) against one launch per step: the
     populations and fields bit for bit, the force records to rounding (summed per tile instead
-    of per row block), for step counts that leave every remainder of the 5-step launches, on a
-    disk and on a mask that touches the inlet and outlet columns and the walls; with a record
-    every step, one call of 700 steps holds more records than the per-tile partials' slots, so
-    they are summed in several batches within the call."""
+    of per row block), for step counts that leave several remainders of the launches (7 steps
+    by default), on a disk and on a mask that touches the inlet and outlet columns and the
+    walls; with a record every step, one call of 700 steps holds more records than the
+    per-tile partials' slots, so they are summed in several batches within the call."""
     _tb_vs_one_launch(pkg, monkeypatch, fi, case)
 
 
-@pytest.mark.parametrize("tile,depth", [("32", "5"), ("24", "3"), ("32", "8"), ("24", "1")])
+@pytest.mark.parametrize("tile,depth", [("24", "5"), ("32", "5"), ("24", "3"), ("32", "8"), ("24", "1"), ("32", "10")])
 def test_temporal_blocking_tile_and_depth_variants(pkg, monkeypatch, tile, depth):
     """The same comparison for the other tile sides and depths the host can select (the 32 x 32
     tile is the default where 24 x 24 tiles would not fit the CUs once): each step computes

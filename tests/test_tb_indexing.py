@@ -12,8 +12,9 @@ K = 1..10 with a core of at least 2 x 2 nodes, and every launch length up to K):
   * the core store reads back exactly the slots the core's threads wrote;
   * padding the LDS rows to T + 1 cuts the bank conflicts of the pull reads (modelled with 64
     banks of 4 bytes, 32 lanes per cycle);
-  * the busiest SIMD of the default 24 x 24, K = 5 tile runs 8 wave-steps per launch, against
-    15 when every wave runs every step (waves assigned to the 4 SIMDs in turn)."""
+  * the busiest SIMD of a 24 x 24, K = 5 tile runs 8 wave-steps per launch, against 15 when
+    every wave runs every step (waves assigned to the 4 SIMDs in turn); 14 against 28 for the
+    28 x 28, K = 7 tile."""
 import numpy as np
 import pytest
 
@@ -131,8 +132,11 @@ def test_padded_rows_conflict_less():
     assert (cost[T + 1], cost[T]) == (160, 240)
 
 
-def test_busiest_simd_wave_steps_default_tile():
-    T, K = 24, 5
+@pytest.mark.parametrize("T,K,expect", [(24, 5, (8, 15)), (28, 7, (14, 28))])
+def test_busiest_simd_wave_steps(T, K, expect):
+    """Wave-steps of the busiest SIMD per launch, ring order against every tile node's wave in
+    every step: the 24 x 24 tile at depth 5 and the 28 x 28 tile at depth 7 (the default for a
+    400 x 100 domain)."""
     W = T - 2 * K
     nwaves = (T * T + 63) // 64    # one thread per tile node, every wave every step
 
@@ -144,4 +148,92 @@ def test_busiest_simd_wave_steps_default_tile():
 
     ring_order = sum(busiest(range((((W + 2 * (K - 1 - s)) ** 2) + 63) // 64)) for s in range(K))
     every_wave = K * busiest(range(nwaves))
-    assert (ring_order, every_wave) == (8, 15)
+    assert (ring_order, every_wave) == expect
+
+
+def ghost_check(nx, ny, solid, T, K, steps):
+    """The write phase of every step after the first (live nodes write their own position, the
+    source position of each bounce-back link, and at the outlet the position beyond it) writes
+    no LDS position twice, and every pull of a live computed node then reads what the boundary
+    rules take: its source's own value, the node's own opposite value (wall or solid source),
+    the outlet neighbour's value (zero gradient), or a position outside the domain that no node
+    writes (the inlet's unknowns, which read 0)."""
+    W = T - 2 * K
+    m = tile_map(T, K)
+    tiles_x, tiles_y = -(-nx // W), -(-ny // W)
+    for b in range(tiles_x * tiles_y):
+        X0, Y0 = (b % tiles_x) * W - K, (b // tiles_x) * W - K
+
+        def inside(x, y):
+            return 0 <= x < nx and 0 <= y < ny
+
+        def live(x, y):
+            return inside(x, y) and not solid[y, x]
+
+        for s in range(1, steps):
+            rs = steps - 1 - s
+            ncomp, nsrc = (W + 2 * rs) ** 2, (W + 2 * rs + 2) ** 2
+            writes = {}
+            for t in range(nsrc):
+                tx, ty = m[t]
+                X, Y = X0 + tx, Y0 + ty
+                if not live(X, Y):
+                    continue
+                for i in range(9):
+                    key = (i, tx, ty)
+                    assert key not in writes
+                    writes[key] = ("own", X, Y, i)
+                for i, (ex, ey) in enumerate(E9):
+                    if i == 0:
+                        continue
+                    zh = ex > 0 and X == 0
+                    wall = not zh and not 0 <= Y - ey < ny
+                    sx = X - ex
+                    if ex < 0 and X == nx - 1:
+                        sx = nx - 1
+                    sol = not zh and not wall and solid[Y - ey, sx]
+                    if wall or sol:
+                        key = (i, tx - ex, ty - ey)
+                        assert key not in writes
+                        writes[key] = ("own", X, Y, E9.index((-ex, -ey)))
+                    if X == nx - 1 and ex < 0:
+                        key = (i, tx + 1, ty)
+                        assert key not in writes
+                        writes[key] = ("own", X, Y, i)
+            for t in range(ncomp):
+                tx, ty = m[t]
+                X, Y = X0 + tx, Y0 + ty
+                if not live(X, Y):
+                    continue
+                for i, (ex, ey) in enumerate(E9):
+                    got = writes.get((i, tx - ex, ty - ey))
+                    zh = ex > 0 and X == 0
+                    if zh:
+                        assert got is None and not inside(X - ex, Y - ey)
+                        continue
+                    if i > 0 and not 0 <= Y - ey < ny:
+                        assert got == ("own", X, Y, E9.index((-ex, -ey)))
+                        continue
+                    sx = X - ex
+                    if ex < 0 and X == nx - 1:
+                        sx = nx - 1
+                    if solid[Y - ey, sx]:
+                        assert got == ("own", X, Y, E9.index((-ex, -ey)))
+                    else:
+                        assert got == ("own", sx, Y - ey, i)
+
+
+@pytest.mark.parametrize("T,K", [(24, 5), (28, 7), (24, 3)])
+def test_ghost_positions_on_a_mask_touching_every_boundary(T, K):
+    nx, ny = 100, 40
+    solid = np.zeros((ny, nx), bool)
+    solid[2:5, 0] = True            # inlet column
+    solid[ny // 2, 1] = True
+    solid[3:6, nx - 1] = True       # outlet column
+    solid[ny - 4, nx - 2] = True
+    solid[0, 40:43] = True          # against the y = 0 wall
+    solid[ny // 2, 75] = True
+    solid[ny // 2 + 1, 50:54] = True
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    solid |= (xx - 25) ** 2 + (yy - 19.5) ** 2 < 25
+    ghost_check(nx, ny, solid, T, K, K)
