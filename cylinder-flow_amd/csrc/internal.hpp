@@ -94,6 +94,9 @@ hipError_t fast_step_f16(int Q, int V, const StepBufs &b, bool force, bool mac, 
 hipError_t fast_step_bf16(int Q, int V, const StepBufs &b, bool force, bool mac, const FusedArgs &a, hipStream_t s);
 // several D2Q9 FP64 steps per launch (small_kernels.hip): tb_depth() steps at most, T x T tiles
 // (tb_tile), tb_tiles(g, T, K) blocks with tb_lds_bytes(T) of LDS each; tb_prepare() before use
+// bounce-back links of a node for k_tb2d: bit i-1 when direction i pulls from a solid node
+// (after the outlet's clamp), bit 7+i when it pulls across a wall (not for the inlet's unknowns)
+std::vector<uint16_t> tb_links(const Geo &g, const std::vector<uint8_t> &flags);
 int tb_depth();
 size_t tb_lds_bytes(int T);
 int tb_tiles(const Geo &g, int T, int K, int *tiles_x);
@@ -103,7 +106,7 @@ hipError_t tb_prepare();
 // slots pend, pend + 1, ... of `partials` (tb_records() slots of ntiles * 3 doubles); *nrec = their
 // count.  tb_force_flush sums npend slots into hist[rec0 ...], the first at step step1.
 int tb_records(int K);
-hipError_t tb_steps(const void *A, void *B, const uint8_t *flags, const double *uin, double *partials, int pend,
+hipError_t tb_steps(const void *A, void *B, const uint8_t *flags, const uint16_t *lk, const double *uin, double *partials, int pend,
                     int *nrec, const FusedArgs &a, int T, int K, int steps, long step0, int fi, hipStream_t s);
 hipError_t tb_force_flush(const double *partials, const Geo &g, int T, int K, double *hist, int rec0, int npend,
                           long step1, int fi, hipStream_t s);
@@ -204,6 +207,7 @@ struct lbm_ctx {
     int tb_k = 0;                   // D2Q9 FP64 small domains: steps per temporally blocked launch (0: off)
     int tb_t = 0;                   // and its tile side
     double *tb_partials = nullptr;  // their per-tile force partials (tb_records(tb_k) record slots)
+    uint16_t *tb_lk = nullptr;      // and each node's bounce-back links (tb_links)
     uint32_t *fix = nullptr;        // FP32/16-bit: nodes recomputed per node after the packed path
     uint32_t *fixlk = nullptr;      // their link words
     int nfix = 0;

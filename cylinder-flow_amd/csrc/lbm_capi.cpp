@@ -93,7 +93,7 @@ extern "C" const char *lbm_last_error(const lbm_ctx *ctx) { return ctx ? ctx->er
 static void free_all(lbm_ctx *c) {
     void *ptrs[] = {c->A, c->B, c->flags, c->dsfrac, c->duin, c->macros, c->block_slot, c->partials, c->hist, c->fix,
                     c->fixlk, c->rowwin, c->wlk, c->link, c->linklk, c->link_slot, c->rowclean, c->rf, c->rft, c->rmac, c->rprof, c->rusr, c->rfpre, c->rsurf,
-                    c->snap_dev, c->dbad, c->tb_partials};
+                    c->snap_dev, c->dbad, c->tb_partials, c->tb_lk};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c->A = c->B = nullptr;
@@ -111,6 +111,7 @@ static void free_all(lbm_ctx *c) {
     c->snap_dev = nullptr;
     c->dbad = nullptr;
     c->tb_partials = nullptr;
+    c->tb_lk = nullptr;
     c->tb_k = 0;
     c->rf = c->rft = c->rmac = c->rprof = c->rfpre = nullptr;
     c->rsurf = nullptr;
@@ -604,12 +605,15 @@ static int init_plain(lbm_ctx *c) {
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
             tb_prepare() == hipSuccess) {
             const int K = tb_depth(), T = tb_tile(g, K, cus);
-            if ((size_t)lds_max >= tb_lds_bytes(T) + 512) {
+            if ((size_t)lds_max >= tb_lds_bytes(T) + 4096 + 512) {   // + the kernel's static force records
                 c->tb_k = K;
                 c->tb_t = T;
                 int tiles_x = 0;
                 const int nt = tb_tiles(g, T, K, &tiles_x);
                 HIP_TRY(c, hipMalloc(&c->tb_partials, sizeof(double) * 3 * (size_t)nt * tb_records(K)));
+                const std::vector<uint16_t> lk = tb_links(g, flags);
+                HIP_TRY(c, hipMalloc(&c->tb_lk, sizeof(uint16_t) * lk.size()));
+                HIP_TRY(c, hipMemcpy(c->tb_lk, lk.data(), sizeof(uint16_t) * lk.size(), hipMemcpyHostToDevice));
             }
         }
     }
@@ -828,7 +832,7 @@ static int step_n(lbm_ctx *c, int n, hipEvent_t *evs = nullptr) {
             step1 = (c->step + fi) / fi * fi;   // the first force step after the current one
         }
         int nr = 0;
-        HIP_TRY(c, tb_steps(c->A, c->B, c->flags, c->duin, c->tb_partials, pend, &nr, a, c->tb_t, c->tb_k, kk, c->step, fi,
+        HIP_TRY(c, tb_steps(c->A, c->B, c->flags, c->tb_lk, c->duin, c->tb_partials, pend, &nr, a, c->tb_t, c->tb_k, kk, c->step, fi,
                             c->stream));
         pend += nr;
         c->nrec += nr;

@@ -26,7 +26,7 @@ constexpr int TB_KMAX = 10;   // steps per launch at most (tb_depth)
 template <int TB_T, bool ANY_FORCE>
 __global__ void __launch_bounds__(1024)
     k_tb2d(const double *__restrict__ A, double *__restrict__ B, const uint8_t *__restrict__ flags,
-           const double *__restrict__ uin, double *__restrict__ partials, const FusedArgs a, int K, int steps,
+           const uint16_t *__restrict__ lk, const double *__restrict__ uin, double *__restrict__ partials, const FusedArgs a, int K, int steps,
            long step0, int fi, int tiles_x, int ntiles, int pend) {
     using L = Lat<9>;
     // one thread per node of the rings a step can compute (within K - 1 of the core); the
@@ -37,10 +37,11 @@ __global__ void __launch_bounds__(1024)
     const int W = TB_T - 2 * K;          // core side
     // LDS: two halves of Q x TB_T x TP doubles (step s reads half s % 2; node (tx, ty) at
     // tx * TP + ty, rows padded to TP = TB_T + 1 so that lanes TP apart hit different banks),
-    // then the tile's flags (TB_T x TP bytes)
+    // then the tile's link words (TB_T x TP) and flags (TB_T x TP bytes)
     constexpr int TP = TB_T + 1, HALF = Q * TB_T * TP;
     extern __shared__ double lds[];
-    uint8_t *lfl = (uint8_t *)(lds + 2 * HALF);
+    uint16_t *llk = (uint16_t *)(lds + 2 * HALF);
+    uint8_t *lfl = (uint8_t *)(llk + TB_T * TP);
     // Threads take the tile's nodes ring by ring from the core outwards: threads [0, W*W) the
     // core (y fastest), then ring r = 1..K-1 (the nodes at Chebyshev distance r from the core: its
     // left and right columns, then its top and bottom rows).  The nodes a step must compute, the
@@ -79,19 +80,29 @@ __global__ void __launch_bounds__(1024)
     const bool act = X >= 0 && X < g.nx && Y >= 0 && Y < g.ny;
     const bool own = act && tx >= K && tx < K + W && ty >= K && ty < K + W;
     const double uy = uin[act ? Y : 0];
-    // the tile's flags and populations, x fastest (coalesced), into the LDS (populations into
-    // half 0, which the first step reads): node t, t + NT, ... of the tile's T*T; every load is
-    // issued (from node (0, 0) for positions outside the domain) before the first is used
+    // Every pull is a plain LDS read at the source's position (no boundary rule in the pull):
+    // each live node also writes, with its own values, the values its rules pull from
+    // positions no live node owns: h_opp(i) into the source position of each bounce-back link i
+    // (a wall or a solid source, from its link word: only this node pulls direction i from
+    // there), and at the outlet (x = nx - 1) its values of the directions with e_x < 0 into the
+    // position beyond the outlet (x = nx), which the outlet node above or below pulls for the
+    // zero gradient.  The inlet's unknowns read positions left of the inlet, which hold 0.
+    // The tile's link words, flags and populations, x fastest (coalesced), into the LDS (into
+    // half 0 with the first step's ghost values, which the loading thread writes for its node;
+    // half 1's positions outside the domain to 0): node t, t + NT, ... of the tile's T*T, every
+    // load issued (from node (0, 0) for positions outside the domain) before the first is used.
     {
         constexpr int NJ = (TB_T * TB_T + NT - 1) / NT;
         double v[NJ][Q];
         uint8_t f[NJ];
+        uint16_t w[NJ];
 #pragma unroll
         for (int k = 0; k < NJ; k++) {
             const int j = t + k * NT, lx = j % TB_T, ly = j / TB_T, AX = X0 + lx, AY = Y0 + ly;
             const bool in = j < TB_T * TB_T && AX >= 0 && AX < g.nx && AY >= 0 && AY < g.ny;
-            const uint32_t ax = in ? (uint32_t)AX : 0u, ay = in ? (uint32_t)AY : 0u;
-            f[k] = flags[ax + (uint32_t)g.nxp * ay];
+            const uint32_t ax = in ? (uint32_t)AX : 0u, ay = in ? (uint32_t)AY : 0u, node = ax + (uint32_t)g.nxp * ay;
+            f[k] = flags[node];
+            w[k] = lk[node];
             static_for<0, Q>([&](auto I) { v[k][I] = A[pel(g, I, ax, ay)]; });
         }
 #pragma unroll
@@ -99,45 +110,38 @@ __global__ void __launch_bounds__(1024)
             const int j = t + k * NT, lx = j % TB_T, ly = j / TB_T, AX = X0 + lx, AY = Y0 + ly;
             const bool in = AX >= 0 && AX < g.nx && AY >= 0 && AY < g.ny;
             if (j < TB_T * TB_T) {
-                lfl[lx * TP + ly] = in ? f[k] : (uint8_t)FL_SOLID;
-                static_for<0, Q>([&](auto I) { lds[I * TB_T * TP + lx * TP + ly] = in ? v[k][I] : 0.0; });
-                // positions outside the domain read 0 in both halves (the inlet's unknowns)
-                if (!in) static_for<0, Q>([&](auto I) { lds[HALF + I * TB_T * TP + lx * TP + ly] = 0.0; });
+                const int pos = lx * TP + ly;
+                const bool lv = in && f[k] != FL_SOLID;
+                lfl[pos] = in ? f[k] : (uint8_t)FL_SOLID;
+                llk[pos] = lv ? w[k] : (uint16_t)0;
+                if (lv) {
+                    static_for<0, Q>([&](auto I) { lds[I * TB_T * TP + pos] = v[k][I]; });
+                    const uint32_t bbk = ((uint32_t)(w[k] & 0xFFu) | (uint32_t)(w[k] >> 8)) << 1;
+                    if (bbk != 0u)
+                        static_for<1, Q>([&](auto I) {
+                            constexpr int i = I;
+                            constexpr int ex = L::ex(i), ey = L::ey(i), o = L::opp(i);
+                            // positions beyond the tile feed only its outermost ring, never computed
+                            if (((bbk >> i) & 1u) && (unsigned)(lx - ex) < (unsigned)TB_T && (unsigned)(ly - ey) < (unsigned)TB_T)
+                                lds[i * TB_T * TP + pos - ex * TP - ey] = v[k][o];
+                        });
+                    if (AX == g.nx - 1 && lx + 1 < TB_T)
+                        static_for<1, Q>([&](auto I) {
+                            constexpr int i = I;
+                            if constexpr (L::ex(i) < 0) lds[i * TB_T * TP + pos + TP] = v[k][i];
+                        });
+                }
+                if (AX < 0) static_for<0, Q>([&](auto I) { lds[I * TB_T * TP + pos] = 0.0; });
+                if (!in) static_for<0, Q>([&](auto I) { lds[HALF + I * TB_T * TP + pos] = 0.0; });
             }
         }
     }
     __syncthreads();
     const uint8_t fl = lfl[slot];
     const bool live = fl != FL_SOLID;
-    // sources that are solid (mask bounce-back), classified once as the pull does (the sources
-    // of rings within K - 1 lie in the tile)
-    uint32_t solid_src = 0;
-    if (fl == FL_NEAR) {
-        static_for<1, Q>([&](auto I) {
-            constexpr int i = I;
-            constexpr int ex = L::ex(i), ey = L::ey(i);
-            if (ex > 0 && X == 0) return;                        // Zou-He unknown
-            const int sy = Y - ey;
-            if ((unsigned)sy >= (unsigned)g.ny) return;          // wall
-            int sx = X - ex;
-            if (ex < 0 && X == g.nx - 1) sx = g.nx - 1;          // outlet clamp
-            if (lfl[(tx + sx - X) * TP + ty - ey] == FL_SOLID) solid_src |= 1u << i;
-        });
-    }
-    // From the second step on, every pull is a plain LDS read at the source's position (no
-    // boundary rule in the pull): each live node also writes, with its own values, the values
-    // its rules pull from positions no live node owns: h_opp(i) into the source position of
-    // each bounce-back link i (a wall or a solid source: only this node pulls direction i from
-    // there), and at the outlet (x = nx - 1) its values of the directions with e_x < 0 into the
-    // position beyond the outlet (x = nx), which the outlet node above or below pulls for the
-    // zero gradient.  The inlet's unknowns read positions outside the domain, which hold 0.
-    uint32_t bb = 0;
-    static_for<1, Q>([&](auto I) {
-        constexpr int i = I;
-        constexpr int ex = L::ex(i), ey = L::ey(i);
-        if (!(ex > 0 && X == 0) && (unsigned)(Y - ey) >= (unsigned)g.ny) bb |= 1u << i;
-    });
-    bb = live ? (bb | solid_src) : 0u;
+    const uint32_t lkw = llk[slot];
+    const uint32_t solid_src = (lkw & 0xFFu) << 1;                 // mask links (momentum exchange)
+    const uint32_t bb = solid_src | ((lkw >> 8) << 1);              // every bounce-back link
     const bool outl = live && X == g.nx - 1;
     double h[Q];
     static_for<0, Q>([&](auto I) { h[I] = lds[I * TB_T * TP + slot]; });
@@ -152,7 +156,8 @@ __global__ void __launch_bounds__(1024)
         // of step s + 1 into the same half
         double *buf = lds + (size_t)(s & 1) * HALF;
         // this step computes the rings within rs = steps - 1 - s of the core, from the values of
-        // the rings within rs + 1 (computed by the previous step, or loaded into half 0)
+        // the rings within rs + 1 (computed by the previous step, or loaded into half 0 with the
+        // ghost values)
         const int rs = steps - 1 - s, ncomp = (W + 2 * rs) * (W + 2 * rs), nsrc = (W + 2 * rs + 2) * (W + 2 * rs + 2);
         if (s > 0) {
             if (t < nsrc && live) {
@@ -174,32 +179,10 @@ __global__ void __launch_bounds__(1024)
         double fx = 0.0, fy = 0.0;
         if (t < ncomp) {
             double gv[Q];
-            if (s > 0) {
-                static_for<0, Q>([&](auto I) {
-                    constexpr int i = I;
-                    gv[i] = buf[i * TB_T * TP + slot - L::ex(i) * TP - L::ey(i)];
-                });
-            } else {
-            // the first step pulls from the loaded tile with the rules
             static_for<0, Q>([&](auto I) {
                 constexpr int i = I;
-                constexpr int ex = L::ex(i), ey = L::ey(i), o = L::opp(i);
-                const bool zh = ex > 0 && X == 0;
-                const int sy = Y - ey;
-                const bool wall = !zh && (unsigned)sy >= (unsigned)g.ny;
-                const bool solid = (solid_src >> i) & 1u;
-                int sx = X - ex;
-                if (ex < 0 && X == g.nx - 1) sx = g.nx - 1;
-                // tile coordinates of the source (within ring rs + 1, so inside the tile; the clamp
-                // only keeps every LDS index in range)
-                int stx = tx + (sx - X), sty = ty - ey;
-                stx = stx < 0 ? 0 : (stx >= TB_T ? TB_T - 1 : stx);
-                sty = sty < 0 ? 0 : (sty >= TB_T ? TB_T - 1 : sty);
-                const double pulled = buf[i * TB_T * TP + stx * TP + sty];
-                const double v = (wall || solid) ? h[o] : pulled;
-                gv[i] = (live && !zh) ? v : 0.0;
+                gv[i] = buf[i * TB_T * TP + slot - L::ex(i) * TP - L::ey(i)];
             });
-            }
             // momentum exchange of the mask links, in direction order, from the pulled values
             // (before the inlet rule), outside the pull so the other steps run without it
             if (ANY_FORCE && force && solid_src != 0u) {
@@ -307,8 +290,9 @@ int tb_depth() {
     return 7;
 }
 
-// two population halves of 9 x T x (T + 1) doubles and the tile's flags (T x (T + 1) bytes)
-size_t tb_lds_bytes(int T) { return sizeof(double) * 2 * 9 * T * (T + 1) + (size_t)T * (T + 1); }
+// two population halves of 9 x T x (T + 1) doubles, the tile's link words and flags (T x (T + 1)
+// of each)
+size_t tb_lds_bytes(int T) { return sizeof(double) * 2 * 9 * T * (T + 1) + (size_t)3 * T * (T + 1); }
 
 int tb_tiles(const Geo &g, int T, int K, int *tiles_x) {
     const int W = T - 2 * K;
@@ -352,6 +336,30 @@ hipError_t tb_prepare() {
     return e;
 }
 
+std::vector<uint16_t> tb_links(const Geo &g, const std::vector<uint8_t> &flags) {
+    using L = Lat<9>;
+    std::vector<uint16_t> lk(flags.size(), 0);
+    for (int y = 0; y < g.ny; y++)
+        for (int x = 0; x < g.nx; x++) {
+            const size_t node = (size_t)x + (size_t)g.nxp * y;
+            if (flags[node] == FL_SOLID) continue;
+            uint16_t b = 0;
+            for (int i = 1; i < 9; i++) {
+                const int ex = L::ex(i), ey = L::ey(i);
+                if (ex > 0 && x == 0) continue;                            // Zou-He unknown
+                const int sy = y - ey;
+                if (sy < 0 || sy >= g.ny) {                                // wall
+                    b |= (uint16_t)(1u << (7 + i));
+                    continue;
+                }
+                const int sx = (ex < 0 && x == g.nx - 1) ? x : x - ex;     // outlet clamp
+                if (flags[(size_t)sx + (size_t)g.nxp * sy] == FL_SOLID) b |= (uint16_t)(1u << (i - 1));
+            }
+            lk[node] = b;
+        }
+    return lk;
+}
+
 // record slots of the partials buffer: the force records of several launches are summed by one
 // k_tb_force launch (a launch of its own costs about as much as a step)
 int tb_records(int K) { return K > 128 ? K : 128; }
@@ -359,7 +367,7 @@ int tb_records(int K) { return K > 128 ? K : 128; }
 // `steps` (<= K) steps from A into B, the first being step0 + 1; the partials of the force records
 // among them go to record slots pend, pend + 1, ... (the caller keeps pend + steps within
 // tb_records(K) and sums them with tb_force_flush).  Returns the number of records.
-hipError_t tb_steps(const void *A, void *B, const uint8_t *flags, const double *uin, double *partials, int pend,
+hipError_t tb_steps(const void *A, void *B, const uint8_t *flags, const uint16_t *lk, const double *uin, double *partials, int pend,
                     int *nrec, const FusedArgs &a, int T, int K, int steps, long step0, int fi, hipStream_t s) {
     int tiles_x = 0;
     const int ntiles = tb_tiles(a.g, T, K, &tiles_x);
@@ -369,7 +377,7 @@ hipError_t tb_steps(const void *A, void *B, const uint8_t *flags, const double *
     if (pend < 0 || pend + nforce > tb_records(K)) return hipErrorInvalidValue;
     if (T - 2 * K < 1 || K > TB_KMAX || steps < 1 || steps > K) return hipErrorInvalidValue;
     auto go = [&](auto kern) {   // (T - 2)^2 threads: the kernel's thread map assumes exactly that
-        hipLaunchKernelGGL(kern, dim3(ntiles), dim3((T - 2) * (T - 2)), lds, s, (const double *)A, (double *)B, flags, uin,
+        hipLaunchKernelGGL(kern, dim3(ntiles), dim3((T - 2) * (T - 2)), lds, s, (const double *)A, (double *)B, flags, lk, uin,
                            partials, a, K, steps, step0, fi, tiles_x, ntiles, pend);
     };
     switch (T) {

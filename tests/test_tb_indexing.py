@@ -152,12 +152,14 @@ def test_busiest_simd_wave_steps(T, K, expect):
 
 
 def ghost_check(nx, ny, solid, T, K, steps):
-    """The write phase of every step after the first (live nodes write their own position, the
-    source position of each bounce-back link, and at the outlet the position beyond it) writes
-    no LDS position twice, and every pull of a live computed node then reads what the boundary
-    rules take: its source's own value, the node's own opposite value (wall or solid source),
-    the outlet neighbour's value (zero gradient), or a position outside the domain that no node
-    writes (the inlet's unknowns, which read 0)."""
+    """The tile load (every live position's loader writes its values and its node's ghost
+    values inside the tile, positions left of the inlet get 0) and the write phase of every
+    later step (live nodes write their own position, the source position of each bounce-back
+    link, and at the outlet the position beyond it) write no LDS position twice, and every pull
+    of a live computed node then reads what the boundary rules take: its source's own value, the
+    node's own opposite value (wall or solid source), the outlet neighbour's value (zero
+    gradient), or a position left of the inlet (the inlet's unknowns, which read 0: written so
+    by the load, and never written in a later step)."""
     W = T - 2 * K
     m = tile_map(T, K)
     tiles_x, tiles_y = -(-nx // W), -(-ny // W)
@@ -170,12 +172,20 @@ def ghost_check(nx, ny, solid, T, K, steps):
         def live(x, y):
             return inside(x, y) and not solid[y, x]
 
-        for s in range(1, steps):
+        for s in range(0, steps):
             rs = steps - 1 - s
             ncomp, nsrc = (W + 2 * rs) ** 2, (W + 2 * rs + 2) ** 2
             writes = {}
-            for t in range(nsrc):
-                tx, ty = m[t]
+            if s == 0:
+                # the tile load: every position's loader, positions left of the inlet to 0
+                writers = [(lx, ly) for ly in range(T) for lx in range(T)]
+                for lx, ly in writers:
+                    if X0 + lx < 0:
+                        for i in range(9):
+                            writes[(i, lx, ly)] = ("zero",)
+            else:
+                writers = [tuple(m[t]) for t in range(nsrc)]
+            for tx, ty in writers:
                 X, Y = X0 + tx, Y0 + ty
                 if not live(X, Y):
                     continue
@@ -192,11 +202,11 @@ def ghost_check(nx, ny, solid, T, K, steps):
                     if ex < 0 and X == nx - 1:
                         sx = nx - 1
                     sol = not zh and not wall and solid[Y - ey, sx]
-                    if wall or sol:
+                    if (wall or sol) and 0 <= tx - ex < T and 0 <= ty - ey < T:
                         key = (i, tx - ex, ty - ey)
                         assert key not in writes
                         writes[key] = ("own", X, Y, E9.index((-ex, -ey)))
-                    if X == nx - 1 and ex < 0:
+                    if X == nx - 1 and ex < 0 and tx + 1 < T:
                         key = (i, tx + 1, ty)
                         assert key not in writes
                         writes[key] = ("own", X, Y, i)
@@ -209,7 +219,7 @@ def ghost_check(nx, ny, solid, T, K, steps):
                     got = writes.get((i, tx - ex, ty - ey))
                     zh = ex > 0 and X == 0
                     if zh:
-                        assert got is None and not inside(X - ex, Y - ey)
+                        assert got in (None, ("zero",)) and not inside(X - ex, Y - ey)
                         continue
                     if i > 0 and not 0 <= Y - ey < ny:
                         assert got == ("own", X, Y, E9.index((-ex, -ey)))
