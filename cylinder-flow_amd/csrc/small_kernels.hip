@@ -43,37 +43,77 @@ __global__ void __launch_bounds__(1024)
     uint16_t *llk = (uint16_t *)(lds + 2 * HALF);
     uint8_t *lfl = (uint8_t *)(llk + TB_T * TP);
     // Threads take the tile's nodes ring by ring from the core outwards: threads [0, W*W) the
-    // core (y fastest), then ring r = 1..K-1 (the nodes at Chebyshev distance r from the core: its
-    // left and right columns, then its top and bottom rows).  The nodes a step must compute, the
-    // core and the rings within the steps still to come, are then the first (W + 2m)^2
+    // core (y fastest), then ring r = 1..K-1 (the nodes at Chebyshev distance r from the core:
+    // its left and right columns, then its top and bottom rows).  The nodes a step must compute,
+    // the live nodes within the steps still to come of the core, are then the first (W + 2m)^2
     // threads, and the waves beyond them skip the step: the busiest SIMD of a 24 x 24 tile runs
-    // 2+2+2+1+1 wave-steps in 5 steps instead of 3 per step.
+    // 2+2+2+1+1 wave-steps in 5 steps instead of 3 per step.  Tiles at the inlet (their left K
+    // columns lie outside the domain) take the inlet column x = 0 first, so that one wave runs
+    // the inlet's long sequence, then the nodes right of it ring by ring, then those left of it
+    // (never computed).
     const int t = threadIdx.x;
     int tx, ty;
-    if (t < W * W) {
-        tx = K + t / W;
-        ty = K + t % W;
-    } else {
-        int r = 1, side = W + 2;
-        while (t >= side * side) {       // at most K - 1 rings: blockDim.x is NT
-            r++;
-            side += 2;
-        }
-        const int q = t - (side - 2) * (side - 2), lo = K - r, hi = lo + side - 1;
-        if (q < side) {
-            tx = lo;
-            ty = lo + q;
-        } else if (q < 2 * side) {
-            tx = hi;
-            ty = lo + q - side;
-        } else if (q < 3 * side - 2) {
-            ty = lo;
-            tx = lo + 1 + q - 2 * side;
+    if (blockIdx.x % tiles_x != 0) {
+        if (t < W * W) {
+            tx = K + t / W;
+            ty = K + t % W;
         } else {
-            ty = hi;
-            tx = lo + 1 + q - (3 * side - 2);
+            int r = 1, side = W + 2;
+            while (t >= side * side) {   // at most K - 1 rings: blockDim.x is NT
+                r++;
+                side += 2;
+            }
+            const int q = t - (side - 2) * (side - 2), lo = K - r, hi = lo + side - 1;
+            if (q < side) {
+                tx = lo;
+                ty = lo + q;
+            } else if (q < 2 * side) {
+                tx = hi;
+                ty = lo + q - side;
+            } else if (q < 3 * side - 2) {
+                ty = lo;
+                tx = lo + 1 + q - 2 * side;
+            } else {
+                ty = hi;
+                tx = lo + 1 + q - (3 * side - 2);
+            }
+        }
+    } else {
+        int q = t;
+        if (q < TB_T - 2) {                      // the inlet column
+            tx = K;
+            ty = 1 + q;
+        } else if ((q -= TB_T - 2) < (W - 1) * W) {   // the core right of it
+            tx = K + 1 + q / W;
+            ty = K + q % W;
+        } else {
+            q -= (W - 1) * W;
+            int r = 1;
+            while (r < K && q >= 3 * W + 4 * r - 4) {  // ring r right of the inlet column
+                q -= 3 * W + 4 * r - 4;
+                r++;
+            }
+            if (r < K) {
+                if (q < W + 2 * r) {                   // its right column
+                    tx = K + W - 1 + r;
+                    ty = K - r + q;
+                } else if ((q -= W + 2 * r) < W + r - 2) {   // its top row
+                    ty = K - r;
+                    tx = K + 1 + q;
+                } else {                               // its bottom row
+                    ty = K + W - 1 + r;
+                    tx = K + 1 + q - (W + r - 2);
+                }
+            } else {                                   // left of the inlet column
+                tx = 1 + q / (TB_T - 2);
+                ty = 1 + q % (TB_T - 2);
+            }
         }
     }
+    // Chebyshev distance of the node from the core
+    const int dx = tx < K ? K - tx : (tx >= K + W ? tx - (K + W - 1) : 0);
+    const int dy = ty < K ? K - ty : (ty >= K + W ? ty - (K + W - 1) : 0);
+    const int dist = dx > dy ? dx : dy;
     const int slot = tx * TP + ty;       // LDS index of the node
     const int X0 = (int)(blockIdx.x % tiles_x) * W - K, Y0 = (int)(blockIdx.x / tiles_x) * W - K;
     const int X = X0 + tx, Y = Y0 + ty;
@@ -155,12 +195,12 @@ __global__ void __launch_bounds__(1024)
         // two halves: the barrier of step s also orders the reads of step s - 1 before the writes
         // of step s + 1 into the same half
         double *buf = lds + (size_t)(s & 1) * HALF;
-        // this step computes the rings within rs = steps - 1 - s of the core, from the values of
-        // the rings within rs + 1 (computed by the previous step, or loaded into half 0 with the
-        // ghost values)
-        const int rs = steps - 1 - s, ncomp = (W + 2 * rs) * (W + 2 * rs), nsrc = (W + 2 * rs + 2) * (W + 2 * rs + 2);
+        // this step computes the live nodes within rs = steps - 1 - s of the core, from the
+        // values of those within rs + 1 (computed by the previous step, or loaded into half 0
+        // with the ghost values)
+        const int rs = steps - 1 - s;
         if (s > 0) {
-            if (t < nsrc && live) {
+            if (dist <= rs + 1 && live) {
                 static_for<0, Q>([&](auto I) { buf[I * TB_T * TP + slot] = h[I]; });
                 if (bb != 0u)
                     static_for<1, Q>([&](auto I) {
@@ -177,7 +217,7 @@ __global__ void __launch_bounds__(1024)
             __syncthreads();
         }
         double fx = 0.0, fy = 0.0;
-        if (t < ncomp) {
+        if (dist <= rs && live) {
             double gv[Q];
             static_for<0, Q>([&](auto I) {
                 constexpr int i = I;
@@ -210,7 +250,7 @@ __global__ void __launch_bounds__(1024)
             static_for<0, Q>([&](auto I) {
                 constexpr int i = I;
                 const double feq = feq_ref<Q, i>(p, u, v, w);
-                h[i] = gv[i] - div_cr(gv[i] - feq, a.tau, a.rtau);   // (a solid node's values are never used)
+                h[i] = gv[i] - div_cr(gv[i] - feq, a.tau, a.rtau);
             });
         }
         if (ANY_FORCE && force) {   // uniform over the grid
@@ -230,7 +270,7 @@ __global__ void __launch_bounds__(1024)
     // the core through the LDS half the last step did not read (its last reader, step
     // steps - 2, finished before the barrier of step steps - 1), then stored x fastest
     double *out = lds + (size_t)(steps & 1) * HALF;
-    if (t < W * W) static_for<0, Q>([&](auto I) { out[I * TB_T * TP + slot] = h[I]; });
+    if (dist == 0) static_for<0, Q>([&](auto I) { out[I * TB_T * TP + slot] = h[I]; });
     __syncthreads();
     if (ANY_FORCE && t < kr) {
         // record t of this launch into slot pend + t: the waves' sums in wave order

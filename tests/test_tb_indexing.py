@@ -48,6 +48,45 @@ def tile_map(T, K):
     return np.array(out)
 
 
+def inlet_map(T, K):
+    """(tx, ty) of thread t in a tile at the inlet, as k_tb2d computes it: the inlet column
+    (tx = K), the core right of it, ring r = 1..K-1 right of it (right column, top row, bottom
+    row), then the positions left of the inlet column."""
+    W = T - 2 * K
+    out = []
+    for t in range((T - 2) ** 2):
+        q = t
+        if q < T - 2:
+            out.append((K, 1 + q))
+            continue
+        q -= T - 2
+        if q < (W - 1) * W:
+            out.append((K + 1 + q // W, K + q % W))
+            continue
+        q -= (W - 1) * W
+        r = 1
+        while r < K and q >= 3 * W + 4 * r - 4:
+            q -= 3 * W + 4 * r - 4
+            r += 1
+        if r < K:
+            if q < W + 2 * r:
+                out.append((K + W - 1 + r, K - r + q))
+            elif q - (W + 2 * r) < W + r - 2:
+                out.append((K + 1 + q - (W + 2 * r), K - r))
+            else:
+                out.append((K + 1 + q - (W + 2 * r) - (W + r - 2), K + W - 1 + r))
+        else:
+            out.append((1 + q // (T - 2), 1 + q % (T - 2)))
+    return np.array(out)
+
+
+def dist(m, T, K):
+    W = T - 2 * K
+    dx = np.maximum(np.maximum(K - m[:, 0], m[:, 0] - (K + W - 1)), 0)
+    dy = np.maximum(np.maximum(K - m[:, 1], m[:, 1] - (K + W - 1)), 0)
+    return np.maximum(dx, dy)
+
+
 CASES = [(T, K) for T in (24, 26, 28, 30, 32) for K in range(1, 11) if T - 2 * K >= 2]
 NT = {T: (T - 2) ** 2 for T in (24, 26, 28, 30, 32)}
 
@@ -65,6 +104,28 @@ def test_ring_order_is_a_bijection_with_prefix_regions(T, K):
     for r in range(K):
         n = (W + 2 * r) ** 2
         assert (d[:n] <= r).all() and (d[n:] > r).all()
+
+
+@pytest.mark.parametrize("T,K", CASES)
+def test_inlet_tiles_put_the_inlet_column_in_one_wave(T, K):
+    """The inlet tiles' order is a bijection onto the same positions, its first T - 2 threads
+    (one wave) hold the inlet column, and the positions left of it come last."""
+    W = T - 2 * K
+    m = inlet_map(T, K)
+    assert len(m) == NT[T] and m.min() >= 1 and m.max() < T - 1
+    assert len({(int(x), int(y)) for x, y in m}) == NT[T]
+    assert (m[:T - 2, 0] == K).all() and T - 2 <= 64
+    assert (m[:, 0] == K).sum() == T - 2
+    left = m[:, 0] < K
+    assert not left[:NT[T] - (K - 1) * (T - 2)].any() and left[NT[T] - (K - 1) * (T - 2):].all()
+    # the live positions within m of the core (right of the inlet column) come before the rest
+    d = dist(m, T, K)
+    for r in range(K):
+        sel = (d <= r) & (m[:, 0] > K)
+        n = sel.sum()
+        if n:
+            first = np.flatnonzero(sel)
+            assert first.max() < T - 2 + n, (r, first.max())
 
 
 @pytest.mark.parametrize("T,K", CASES)
@@ -161,10 +222,12 @@ def ghost_check(nx, ny, solid, T, K, steps):
     gradient), or a position left of the inlet (the inlet's unknowns, which read 0: written so
     by the load, and never written in a later step)."""
     W = T - 2 * K
-    m = tile_map(T, K)
+    maps = {False: tile_map(T, K), True: inlet_map(T, K)}
     tiles_x, tiles_y = -(-nx // W), -(-ny // W)
     for b in range(tiles_x * tiles_y):
         X0, Y0 = (b % tiles_x) * W - K, (b // tiles_x) * W - K
+        m = maps[b % tiles_x == 0]
+        d = dist(m, T, K)
 
         def inside(x, y):
             return 0 <= x < nx and 0 <= y < ny
@@ -174,7 +237,6 @@ def ghost_check(nx, ny, solid, T, K, steps):
 
         for s in range(0, steps):
             rs = steps - 1 - s
-            ncomp, nsrc = (W + 2 * rs) ** 2, (W + 2 * rs + 2) ** 2
             writes = {}
             if s == 0:
                 # the tile load: every position's loader, positions left of the inlet to 0
@@ -184,7 +246,7 @@ def ghost_check(nx, ny, solid, T, K, steps):
                         for i in range(9):
                             writes[(i, lx, ly)] = ("zero",)
             else:
-                writers = [tuple(m[t]) for t in range(nsrc)]
+                writers = [tuple(m[t]) for t in range(len(m)) if d[t] <= rs + 1]
             for tx, ty in writers:
                 X, Y = X0 + tx, Y0 + ty
                 if not live(X, Y):
@@ -210,7 +272,9 @@ def ghost_check(nx, ny, solid, T, K, steps):
                         key = (i, tx + 1, ty)
                         assert key not in writes
                         writes[key] = ("own", X, Y, i)
-            for t in range(ncomp):
+            for t in range(len(m)):
+                if d[t] > rs:
+                    continue
                 tx, ty = m[t]
                 X, Y = X0 + tx, Y0 + ty
                 if not live(X, Y):
