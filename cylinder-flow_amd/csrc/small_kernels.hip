@@ -26,8 +26,8 @@ constexpr int TB_KMAX = 10;   // steps per launch at most (tb_depth)
 template <int TB_T, bool ANY_FORCE>
 __global__ void __launch_bounds__(1024)
     k_tb2d(const double *__restrict__ A, double *__restrict__ B, const uint8_t *__restrict__ flags,
-           const uint16_t *__restrict__ lk, const double *__restrict__ uin, double *__restrict__ partials, const FusedArgs a, int K, int steps,
-           long step0, int fi, int tiles_x, int ntiles, int pend) {
+           const uint16_t *__restrict__ lk, const double *__restrict__ uin, double *__restrict__ partials,
+           const FusedArgs a, int K, int steps, long step0, int fi, int tiles_x, int ntiles, int pend) {
     using L = Lat<9>;
     // one thread per node of the rings a step can compute (within K - 1 of the core); the
     // outermost ring is only read, and is loaded with the rest of the tile
@@ -162,7 +162,8 @@ __global__ void __launch_bounds__(1024)
                             constexpr int i = I;
                             constexpr int ex = L::ex(i), ey = L::ey(i), o = L::opp(i);
                             // positions beyond the tile feed only its outermost ring, never computed
-                            if (((bbk >> i) & 1u) && (unsigned)(lx - ex) < (unsigned)TB_T && (unsigned)(ly - ey) < (unsigned)TB_T)
+                            if (((bbk >> i) & 1u) && (unsigned)(lx - ex) < (unsigned)TB_T &&
+                                (unsigned)(ly - ey) < (unsigned)TB_T)
                                 lds[i * TB_T * TP + pos - ex * TP - ey] = v[k][o];
                         });
                     if (AX == g.nx - 1 && lx + 1 < TB_T)
