@@ -311,3 +311,15 @@ def test_ghost_positions_on_a_mask_touching_every_boundary(T, K):
     yy, xx = np.mgrid[0:ny, 0:nx]
     solid |= (xx - 25) ** 2 + (yy - 19.5) ** 2 < 25
     ghost_check(nx, ny, solid, T, K, K)
+
+
+def test_xcd_grouping_is_a_bijection():
+    """Workgroup b takes tile (b % 8) (n / 8) + min(b % 8, n % 8) + b / 8: every tile once, for
+    every tile count, and on 400 x 100 (8 rows of 29 tiles) XCD k takes tile row k."""
+    for n in range(1, 3000):
+        tiles = [(b % 8) * (n // 8) + min(b % 8, n % 8) + b // 8 for b in range(n)]
+        assert sorted(tiles) == list(range(n))
+    n, tiles_x = 232, 29
+    for b in range(n):
+        tile = (b % 8) * (n // 8) + min(b % 8, n % 8) + b // 8
+        assert tile // tiles_x == b % 8
