@@ -313,13 +313,25 @@ def test_ghost_positions_on_a_mask_touching_every_boundary(T, K):
     ghost_check(nx, ny, solid, T, K, K)
 
 
+def xcd_tile(b, ntiles, tiles_x):
+    """Tile of workgroup b in k_tb2d: XCD b % 8 takes a run of consecutive positions of an order
+    that runs down bands of two tile rows column by column."""
+    pos = (b % 8) * (ntiles // 8) + min(b % 8, ntiles % 8) + b // 8
+    band, in_band = pos // (2 * tiles_x), pos % (2 * tiles_x)
+    rows = min(2, ntiles // tiles_x - 2 * band)
+    return (2 * band + in_band % rows) * tiles_x + in_band // rows
+
+
 def test_xcd_grouping_is_a_bijection():
-    """Workgroup b takes tile (b % 8) (n / 8) + min(b % 8, n % 8) + b / 8: every tile once, for
-    every tile count, and on 400 x 100 (8 rows of 29 tiles) XCD k takes tile row k."""
-    for n in range(1, 3000):
-        tiles = [(b % 8) * (n // 8) + min(b % 8, n % 8) + b // 8 for b in range(n)]
-        assert sorted(tiles) == list(range(n))
+    """Every tile once, for every grid of up to 40 x 25 tiles; on 400 x 100 (8 rows of 29 tiles)
+    XCD k holds a block of 2 rows and 14-15 columns."""
+    for tiles_x in range(1, 41):
+        for tiles_y in range(1, 26):
+            n = tiles_x * tiles_y
+            assert sorted(xcd_tile(b, n, tiles_x) for b in range(n)) == list(range(n))
     n, tiles_x = 232, 29
-    for b in range(n):
-        tile = (b % 8) * (n // 8) + min(b % 8, n % 8) + b // 8
-        assert tile // tiles_x == b % 8
+    for k in range(8):
+        tiles = [xcd_tile(b, n, tiles_x) for b in range(k, n, 8)]
+        rows = {t // tiles_x for t in tiles}
+        cols = {t % tiles_x for t in tiles}
+        assert len(tiles) == 29 and len(rows) == 2 and len(cols) in (14, 15)
