@@ -591,14 +591,15 @@ static int init_plain(lbm_ctx *c) {
             HIP_TRY(c, hipMemcpy(c->link_slot, lslot.data(), sizeof(int) * lslot.size(), hipMemcpyHostToDevice));
         }
     }
-    // D2Q9 FP64 on a small domain (population buffer <= 16 MB, where one step costs about a
-    // dependent launch): several steps per launch (temporal blocking, small_kernels.hip), for
-    // walls, inlet, outlet and a mask; LBM_TEMPORAL_BLOCKING=0 / 1 forbids / allows it
-    // regardless of the size.
+    // D2Q9 FP64 on a small domain, where a step costs about a dependent launch: several steps
+    // per launch (temporal blocking, small_kernels.hip), for walls, inlet, outlet and a mask,
+    // when the population buffer is at most 24 MB (measured against one launch per step: 2.3x
+    // at 3 MB, 1.08x at 17.6 MB, even at 34 MB, slower above); LBM_TEMPORAL_BLOCKING=0 / 1
+    // forbids / allows it regardless of the size.
     {
         const char *tbe = std::getenv("LBM_TEMPORAL_BLOCKING");
         const bool eligible = Q == 9 && c->prec == LBM_FP64 && !g.xper && !g.yper && !sf;
-        const bool small = (size_t)8 * Q * g.plane <= ((size_t)16 << 20);
+        const bool small = (size_t)8 * Q * g.plane <= ((size_t)24 << 20);
         int dev = 0, lds_max = 0, cus = 0;
         if (eligible && (tbe ? tbe[0] == '1' : small) && hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
