@@ -335,3 +335,16 @@ def test_xcd_grouping_is_a_bijection():
         rows = {t // tiles_x for t in tiles}
         cols = {t % tiles_x for t in tiles}
         assert len(tiles) == 29 and len(rows) == 2 and len(cols) in (14, 15)
+
+
+@pytest.mark.parametrize("nx,ny", [(12, 6), (30, 9), (14, 14), (15, 3)])
+def test_ghost_positions_on_domains_smaller_than_a_core(nx, ny):
+    """One tile column (every tile is an inlet tile, holding the outlet too) and one or two
+    tile rows (walls on both sides), with a solid node next to the inlet, the outlet and a
+    wall: the write phases stay race-free and every pull reads what the rules take."""
+    solid = np.zeros((ny, nx), bool)
+    solid[ny // 2, 1] = True
+    solid[0, nx // 2] = True
+    solid[ny - 1, nx - 1] = True
+    for T, K in ((28, 7), (24, 5), (24, 1)):
+        ghost_check(nx, ny, solid, T, K, K)
