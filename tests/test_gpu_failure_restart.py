@@ -92,3 +92,35 @@ def test_split_run_restarts_bit_for_bit(pkg, monkeypatch, prec, dense):
     for x, y in zip(ma, mc):
         assert np.array_equal(x, y)
     assert np.array_equal(Fc, Fa[Fa[:, 0] > 10])     # records of steps 12, 15, 18
+
+
+@pytest.mark.parametrize("tb", ["1", "0"])
+def test_split_run_restarts_bit_for_bit_d2q9_fp64(pkg, monkeypatch, tb):
+    """The D2Q9 FP64 plain path of small domains, with several steps per launch (whose force
+    steps within a launch follow from the restored step counter) and with one launch per step:
+    a run split at step 11 and restored into a new context equals the uninterrupted 25 steps."""
+    L = pkg.lbm
+    monkeypatch.setenv("LBM_TEMPORAL_BLOCKING", tb)
+    g = L.make_config(9, 100, 40, 1, tau0=0.06, u0=0.05, precision=L.FP64, bc_mode=L.BC_PLAIN, perturb=1e-3,
+                      force_interval=3, body=L.BODY_SPHERE, center=(25.0, 19.5, 0.0), r2=25.0)
+    print(f"[restart] d2q9 fp64 temporal blocking {tb}", flush=True)
+    with pkg.Solver(g) as a:
+        a.initialize()
+        a.step(25)
+        fa, ma, Fa = a.populations(), a.fields(), a.forces()
+    with pkg.Solver(g) as b:
+        b.initialize()
+        b.step(11)
+        saved, step = b.populations(), b.info().step
+    assert step == 11
+    with pkg.Solver(g) as c:
+        c.initialize()
+        c.set_populations(saved)
+        c.set_step(step)
+        c.step(14)
+        fc, mc, Fc = c.populations(), c.fields(), c.forces()
+        assert c.info().step == 25
+    assert np.array_equal(fa, fc)
+    for x, y in zip(ma, mc):
+        assert np.array_equal(x, y)
+    assert np.array_equal(Fc, Fa[Fa[:, 0] > 11])     # records of steps 12, 15, 18, 21, 24
