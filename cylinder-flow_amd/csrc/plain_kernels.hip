@@ -194,11 +194,25 @@ __global__ void __launch_bounds__(256) k_force_chunks(const double *__restrict__
                                                       double *__restrict__ out) {
     const int lo = blockIdx.x * FORCE_CHUNK, hi = min(nslots, lo + FORCE_CHUNK);
     double a = 0.0, b = 0.0, c = 0.0;
-    for (int j = lo + (int)threadIdx.x; j < hi; j += 256) {
-        a += partials[3 * (size_t)j];
-        b += partials[3 * (size_t)j + 1];
-        c += partials[3 * (size_t)j + 2];
+    // the FORCE_CHUNK / 256 = 16 slots of this thread, all loads in flight before the adds
+    // (which keep the order j = lo + threadIdx.x, + 256, ...)
+    constexpr int PER = FORCE_CHUNK / 256;
+    double va[PER], vb[PER], vc[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int j = lo + (int)threadIdx.x + 256 * k;
+        const size_t m = j < hi ? (size_t)j : (size_t)lo;
+        va[k] = partials[3 * m];
+        vb[k] = partials[3 * m + 1];
+        vc[k] = partials[3 * m + 2];
     }
+#pragma unroll
+    for (int k = 0; k < PER; k++)
+        if (lo + (int)threadIdx.x + 256 * k < hi) {
+            a += va[k];
+            b += vb[k];
+            c += vc[k];
+        }
     __shared__ double red[3 * 4];
     block_sum3<4>(a, b, c, red);
     if (threadIdx.x == 0) {
