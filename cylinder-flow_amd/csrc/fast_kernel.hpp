@@ -356,10 +356,14 @@ __global__ void __launch_bounds__(256)
     const uint32_t node = list[j], lk = links[j];
     const uint32_t row = node / (uint32_t)g.nxp;
     const uint32_t x = node - row * (uint32_t)g.nxp;
+    // every bounced value first, so the loads are in flight together: a direction without a
+    // link loads the node's own rest value instead (a line the wave reads anyway), so no load
+    // sits in a branch of its own and waits there
     T val[Q];
-    static_for<1, Q>([&](auto I) {   // every bounced value first, so the loads are in flight together
+    const size_t self0 = pel(g, 0, x, row);
+    static_for<1, Q>([&](auto I) {
         constexpr int i = I;
-        if ((lk >> i) & 1u) val[i] = A[pel(g, L::opp(i), x, row)];
+        val[i] = A[((lk >> i) & 1u) ? pel(g, L::opp(i), x, row) : self0];
     });
     double fx = 0.0, fy = 0.0, fz = 0.0;
     static_for<1, Q>([&](auto I) {
