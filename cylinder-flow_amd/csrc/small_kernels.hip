@@ -194,8 +194,9 @@ __global__ void __launch_bounds__(1024)
     const uint32_t solid_src = (lkw & 0xFFu) << 1;                 // mask links (momentum exchange)
     const uint32_t bb = solid_src | ((lkw >> 8) << 1);              // every bounce-back link
     const bool outl = live && X == g.nx - 1;
-    double h[Q];
-    static_for<0, Q>([&](auto I) { h[I] = lds[I * TB_T * TP + slot]; });
+    // a node's values in registers: every step pulls from the LDS, and a node writes (from the
+    // second step on, and into the core store) only values a step of its own computed
+    double h[Q] = {};
     // force records: the steps s = sf, sf + fi, ... of this launch (step0 + s + 1 a multiple of
     // fi); each wave's sums go to red[record][wave] and are added, waves in order, after the
     // launch's last barrier, so no force step waits for a block-wide sum
