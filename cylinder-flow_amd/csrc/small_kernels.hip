@@ -355,7 +355,7 @@ int tb_tiles(const Geo &g, int T, int K, int *tiles_x) {
 // the tile sides instantiated, and the tile side for a domain: the smallest whose tiles fit the
 // CUs once and leave a core of at least 2 x 2 nodes (deeper launches need wider halos), else
 // the largest; LBM_TB_TILE overrides with any instantiated side that leaves a 2 x 2 core
-static const int kTbSides[] = {24, 26, 28, 30, 32};
+static const int kTbSides[] = {24, 25, 26, 27, 28, 29, 30, 31, 32};
 
 int tb_tile(const Geo &g, int K, int cus) {
     if (const char *e = std::getenv("LBM_TB_TILE")) {
@@ -381,9 +381,13 @@ static hipError_t tb_attr() {
 
 hipError_t tb_prepare() {
     hipError_t e = tb_attr<24>();
+    if (e == hipSuccess) e = tb_attr<25>();
     if (e == hipSuccess) e = tb_attr<26>();
+    if (e == hipSuccess) e = tb_attr<27>();
     if (e == hipSuccess) e = tb_attr<28>();
+    if (e == hipSuccess) e = tb_attr<29>();
     if (e == hipSuccess) e = tb_attr<30>();
+    if (e == hipSuccess) e = tb_attr<31>();
     if (e == hipSuccess) e = tb_attr<32>();
     return e;
 }
@@ -434,9 +438,13 @@ hipError_t tb_steps(const void *A, void *B, const uint8_t *flags, const uint16_t
     };
     switch (T) {
     case 24: nforce > 0 ? go(k_tb2d<24, true>) : go(k_tb2d<24, false>); break;
+    case 25: nforce > 0 ? go(k_tb2d<25, true>) : go(k_tb2d<25, false>); break;
     case 26: nforce > 0 ? go(k_tb2d<26, true>) : go(k_tb2d<26, false>); break;
+    case 27: nforce > 0 ? go(k_tb2d<27, true>) : go(k_tb2d<27, false>); break;
     case 28: nforce > 0 ? go(k_tb2d<28, true>) : go(k_tb2d<28, false>); break;
+    case 29: nforce > 0 ? go(k_tb2d<29, true>) : go(k_tb2d<29, false>); break;
     case 30: nforce > 0 ? go(k_tb2d<30, true>) : go(k_tb2d<30, false>); break;
+    case 31: nforce > 0 ? go(k_tb2d<31, true>) : go(k_tb2d<31, false>); break;
     case 32: nforce > 0 ? go(k_tb2d<32, true>) : go(k_tb2d<32, false>); break;
     default: return hipErrorInvalidValue;
     }

@@ -504,7 +504,8 @@ def test_temporal_blocking_equals_one_launch_per_step(pkg, monkeypatch, fi, case
     _tb_vs_one_launch(pkg, monkeypatch, fi, case)
 
 
-@pytest.mark.parametrize("tile,depth", [("24", "5"), ("32", "5"), ("24", "3"), ("32", "8"), ("24", "1"), ("32", "10")])
+@pytest.mark.parametrize("tile,depth", [("24", "5"), ("32", "5"), ("24", "3"), ("32", "8"), ("24", "1"), ("32", "10"),
+                                        ("27", "7"), ("25", "4")])
 def test_temporal_blocking_tile_and_depth_variants(pkg, monkeypatch, tile, depth):
     """The same comparison for the other tile sides and depths the host can select (the 32 x 32
     tile is the default where 24 x 24 tiles would not fit the CUs once): each step computes
