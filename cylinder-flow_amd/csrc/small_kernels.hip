@@ -23,6 +23,9 @@ namespace lbm {
 
 constexpr int TB_KMAX = 10;   // steps per launch at most (tb_depth)
 
+// LDS row length (doubles) of a T x T tile: the odd one of T + 1, T + 2
+__host__ __device__ constexpr int tb_row(int T) { return T % 2 == 0 ? T + 1 : T + 2; }
+
 template <int TB_T, bool ANY_FORCE>
 __global__ void __launch_bounds__(1024)
     k_tb2d(const double *__restrict__ A, double *__restrict__ B, const uint8_t *__restrict__ flags,
@@ -36,9 +39,10 @@ __global__ void __launch_bounds__(1024)
     const Geo &g = a.g;
     const int W = TB_T - 2 * K;          // core side
     // LDS: two halves of Q x TB_T x TP doubles (step s reads half s % 2; node (tx, ty) at
-    // tx * TP + ty, rows padded to TP = TB_T + 1 so that lanes TP apart hit different banks),
+    // tx * TP + ty, rows padded to the odd TP = tb_row(TB_T) so that lanes TP apart hit different
+    // banks: an even row of 32 doubles puts every x of a row segment on one bank),
     // then the tile's link words (TB_T x TP) and flags (TB_T x TP bytes)
-    constexpr int TP = TB_T + 1, HALF = Q * TB_T * TP;
+    constexpr int TP = tb_row(TB_T), HALF = Q * TB_T * TP;
     extern __shared__ double lds[];
     uint16_t *llk = (uint16_t *)(lds + 2 * HALF);
     uint8_t *lfl = (uint8_t *)(llk + TB_T * TP);
@@ -342,9 +346,9 @@ int tb_depth() {
     return 7;
 }
 
-// two population halves of 9 x T x (T + 1) doubles, the tile's link words and flags (T x (T + 1)
-// of each)
-size_t tb_lds_bytes(int T) { return sizeof(double) * 2 * 9 * T * (T + 1) + (size_t)3 * T * (T + 1); }
+// two population halves of 9 x T x tb_row(T) doubles, the tile's link words and flags (T x
+// tb_row(T) of each)
+size_t tb_lds_bytes(int T) { return sizeof(double) * 2 * 9 * T * tb_row(T) + (size_t)3 * T * tb_row(T); }
 
 int tb_tiles(const Geo &g, int T, int K, int *tiles_x) {
     const int W = T - 2 * K;
@@ -352,9 +356,8 @@ int tb_tiles(const Geo &g, int T, int K, int *tiles_x) {
     return *tiles_x * ((g.ny + W - 1) / W);
 }
 
-// the tile sides instantiated, and the tile side for a domain: the smallest whose tiles fit the
-// CUs once and leave a core of at least 2 x 2 nodes (deeper launches need wider halos), else
-// the largest; LBM_TB_TILE overrides with any instantiated side that leaves a 2 x 2 core
+// the tile sides instantiated, and the tile side for a domain (leaving a core of at least 2 x 2
+// nodes); LBM_TB_TILE overrides with any instantiated side that leaves a 2 x 2 core
 static const int kTbSides[] = {24, 25, 26, 27, 28, 29, 30, 31, 32};
 
 int tb_tile(const Geo &g, int K, int cus) {
@@ -363,10 +366,19 @@ int tb_tile(const Geo &g, int K, int cus) {
         for (int T : kTbSides)
             if (t == T && t - 2 * K >= 2) return t;
     }
-    int tx = 0;
-    for (int T : kTbSides)
-        if (T - 2 * K >= 2 && tb_tiles(g, T, K, &tx) <= cus) return T;
-    return 32;
+    // the side with the least (rounds of the CUs) x (positions per tile); measured best on
+    // 400 x 100 (27), 600 x 150 (28), 800 x 200 (29) and 1000 x 250 (30) at 7 steps per launch
+    int best = 32, tx = 0;
+    long best_cost = -1;
+    for (int T : kTbSides) {
+        if (T - 2 * K < 2) continue;
+        const long rounds = (tb_tiles(g, T, K, &tx) + cus - 1) / cus, cost = rounds * T * T;
+        if (best_cost < 0 || cost < best_cost) {
+            best = T;
+            best_cost = cost;
+        }
+    }
+    return best;
 }
 
 template <int T>

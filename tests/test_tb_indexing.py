@@ -10,8 +10,9 @@ K = 1..10 with a core of at least 2 x 2 nodes, and every launch length up to K):
     lies in the tile and was written in the same step, or by the coalesced tile load (x
     fastest) before the first step; the mask-link classification reads in-tile flags;
   * the core store reads back exactly the slots the core's threads wrote;
-  * padding the LDS rows to T + 1 cuts the bank conflicts of the pull reads (modelled with 64
-    banks of 4 bytes, 32 lanes per cycle);
+  * padding the LDS rows to an odd length cuts the bank conflicts of the pull reads (modelled
+    with 64 banks of 4 bytes, 32 lanes per cycle), and an even row of 32 doubles would put a
+    row segment on one bank;
   * the busiest SIMD of a 24 x 24, K = 5 tile runs 8 wave-steps per launch, against 15 when
     every wave runs every step (waves assigned to the 4 SIMDs in turn); 14 against 28 for the
     28 x 28, K = 7 tile."""
@@ -131,7 +132,7 @@ def test_inlet_tiles_put_the_inlet_column_in_one_wave(T, K):
 @pytest.mark.parametrize("T,K", CASES)
 def test_each_step_reads_only_slots_written_in_that_step(T, K):
     W = T - 2 * K
-    TP = T + 1
+    TP = T + 1 if T % 2 == 0 else T + 2     # the odd row length (tb_row)
     m = tile_map(T, K)
     slot = m[:, 0] * TP + m[:, 1]
     assert slot.max() < T * TP and len(set(slot)) == NT[T]
@@ -348,3 +349,14 @@ def test_ghost_positions_on_domains_smaller_than_a_core(nx, ny):
     solid[ny - 1, nx - 1] = True
     for T, K in ((28, 7), (24, 5), (24, 1)):
         ghost_check(nx, ny, solid, T, K, K)
+
+
+def test_odd_rows_spread_a_row_segment_over_the_banks():
+    """Lanes walking along x (phase A's writes, the core store's reads, ring rows) are one row
+    apart: with the odd row length every side 24-32 spreads 16 such lanes over at least 16
+    banks, where the even row of 32 doubles (T = 31 with a row of T + 1) puts them on one."""
+    for T in range(24, 33):
+        TP = T + 1 if T % 2 == 0 else T + 2
+        assert TP % 2 == 1
+        assert len({(2 * TP * k) % 64 for k in range(16)}) == 16
+    assert len({(2 * 32 * k) % 64 for k in range(16)}) == 1
