@@ -4,10 +4,10 @@
 // so a step costs what a dependent launch costs (~3.4 us on one MI355X; a captured graph of
 // launches costs the same).  Temporal blocking removes all but one launch per K steps: block b
 // owns a W x W core of nodes and loads its core plus a halo of K nodes on each side (T x T
-// nodes, T = 24..32, the smallest side whose blocks still fit the CUs once) into the LDS,
-// coalesced; each of the K steps pulls from the LDS with the plain-mode boundary rules,
-// collides in registers (one thread per node) and writes the block's populations back to the
-// LDS.  A halo node's result is exact as long as every node it pulled from was, so the exact
+// nodes, T = 24..32, the side with the least rounds of the CUs times T^2) into the LDS,
+// coalesced; each of the K steps pulls from the LDS (boundary rules applied by the values the
+// live nodes write, tb_links) and collides in registers (one thread per node), and the live
+// nodes write their populations back to the LDS.  A halo node's result is exact as long as every node it pulled from was, so the exact
 // region shrinks by one node per step and after K steps covers the core, which is written out;
 // each step computes only that region (threads in ring order, the waves outside it idle).  No
 // block reads another block's output within a launch: there is no grid-wide synchronisation.
